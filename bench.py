@@ -1,0 +1,258 @@
+"""DTC-SpMM benchmark on one MI355X.
+
+Workload (default): the power-law configuration -- R-MAT (a, b, c, d) =
+(0.57, 0.19, 0.19, 0.05), scale 20 (1,048,576 nodes), edge factor 16,
+symmetrised, self loops and duplicates removed, values 1/sqrt(deg_i deg_j)
+float32; B = 1,048,576 x 256 float32 N(0, 1).
+
+A step is one SpMM launch C = A * B over the resident ME-TCF (built once on
+the GPU and timed separately, as the reference times preprocess and SpMM
+apart: DTCSpMM.cpp:337-348 vs DTCSpMM_kernel.cu:841-856).  The variant is
+chosen by exhaustive timing over schedule x precision x chunk among the
+variants whose componentwise error against the float64 CSR product is
+<= 1e-5; the plain FP32 variant is timed the same way for comparison.
+
+Prints one JSON line (see the keys below).  --gpus N > 1 exits: this build
+runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(REPO, "dtc-spmm_asplos24_amd")
+sys.path[:0] = [PKG]
+
+ERR_TARGET = 1e-5
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def timed(stream, fn, iters):
+    import torch
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(iters):
+        fn()
+    e1.record(stream)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / iters
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--scale", type=int, default=20)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--variant", default="auto",
+                    help="auto (exhaustive choice) or schedule:precision:chunk, e.g. balanced:fp32:64")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic-file", default=os.path.join(REPO, "profiles", "traffic_rmat20_n256.json"))
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this DTC-SpMM build runs on one GPU only; multi-GPU runs are not part of it "
+              "(see DESIGN.md for how the path would shard).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from dtc.graphs import rmat
+    from dtc.matrix import DTCMatrix
+    from dtc.metrics import compulsory_bytes, gather_model_bytes, spmm_flops
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream()
+    N = args.n
+    t0 = time.time()
+    a = rmat(args.scale, 16, seed=1)
+    M, nnz = a.num_rows, a.nnz
+    log(f"R-MAT scale {args.scale}: M={M} nnz={nnz} generated in {time.time() - t0:.1f}s")
+
+    # ---- ME-TCF on the GPU (timed apart from the SpMM)
+    rowptr_d = torch.from_numpy(a.rowptr).to(dev)
+    col_d = torch.from_numpy(a.col).to(dev)
+    vals_d = torch.from_numpy(a.values).to(dev)
+    A = DTCMatrix(rowptr_d, col_d, vals_d, num_rows=M, num_cols=M)  # warm-up build
+    torch.cuda.synchronize()
+    pre_ms = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        A = DTCMatrix(rowptr_d, col_d, vals_d, num_rows=M, num_cols=M)
+        torch.cuda.synchronize()
+        pre_ms.append((time.perf_counter() - t) * 1e3)
+    T, W = A.block_count, A.num_windows
+    sum_u = A.sum_unique_columns()
+    log(f"ME-TCF: T={T} W={W} sumU={sum_u} build {np.median(pre_ms):.2f} ms")
+
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(2)
+    B = torch.randn((M, N), generator=gen, device=dev, dtype=torch.float32)
+    C = torch.empty((M, N), device=dev, dtype=torch.float32)
+
+    # ---- float64 reference for the error gate
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+    t = time.time()
+    Bn = B.cpu().numpy()
+    C64, Cabs = oracle.spmm_csr_f64(a.rowptr, a.col, a.values, Bn, M, with_abs=True)
+    log(f"float64 reference in {time.time() - t:.1f}s")
+    mask = Cabs > 0
+
+    def errors(Cd):
+        Ch = Cd.cpu().numpy().astype(np.float64)
+        d = np.abs(Ch - C64)
+        comp = float((d[mask] / Cabs[mask]).max()) if mask.any() else 0.0
+        norm = float(d.max() / np.abs(C64).max())
+        return comp, norm
+
+    def runner(sched, prec, chunk, persistent=False):
+        return lambda: A.spmm(B, schedule=sched, precision=prec, chunk_blocks=chunk,
+                              persistent=persistent, out=C)
+
+    # ---- exhaustive variant timing (the selector's fallback and judge)
+    candidates = []
+    if args.variant == "auto":
+        for prec in ["fp32", "bf16x6", "bf16x3", "fp16", "bf16"]:
+            candidates.append(("windows", prec, 0, False))
+            for chunk in [16, 32, 64, 128, 256]:
+                candidates.append(("balanced", prec, chunk, False))
+            candidates.append(("balanced", prec, 64, True))
+    else:
+        s, p, c = args.variant.split(":")
+        candidates.append((s, p, int(c), False))
+    table = []
+    err_cache = {}
+    for sched, prec, chunk, pers in candidates:
+        fn = runner(sched, prec, chunk, pers)
+        fn()
+        torch.cuda.synchronize()
+        key = prec
+        if key not in err_cache:
+            err_cache[key] = errors(C)
+        ms = timed(stream, fn, 5)
+        table.append(dict(schedule=sched, precision=prec, chunk=chunk, persistent=pers, ms=ms,
+                          gflops=spmm_flops(nnz, N) / ms / 1e6, err=err_cache[key][0],
+                          err_norm=err_cache[key][1]))
+        log(f"  {sched:9s} {prec:7s} chunk={chunk:4d} pers={int(pers)} {ms:8.3f} ms "
+            f"err={err_cache[key][0]:.2e}")
+    fp32_rows = [r for r in table if r["precision"] == "fp32"]
+    best_fp32 = min(fp32_rows, key=lambda r: r["ms"]) if fp32_rows else None
+    ok = [r for r in table if r["err"] <= ERR_TARGET]
+    chosen = min(ok, key=lambda r: r["ms"]) if ok else min(table, key=lambda r: r["ms"])
+
+    # ---- timed region: K steps of the chosen variant
+    fn = runner(chosen["schedule"], chosen["precision"], chosen["chunk"], chosen["persistent"])
+    for _ in range(args.warmup):
+        fn()
+    torch.cuda.synchronize()
+    ms = timed(stream, fn, args.steps)
+    torch.cuda.synchronize()
+    comp_err, norm_err = errors(C)
+
+    flops = spmm_flops(nnz, N)
+    gflops = flops / ms / 1e6
+    bytes_model = gather_model_bytes(sum_u, M, N, nnz, T, W)
+    achieved = bytes_model / (ms * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(args.traffic_file):
+        with open(args.traffic_file) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    fp32_ms = None
+    if best_fp32 is not None:
+        fr = runner(best_fp32["schedule"], "fp32", best_fp32["chunk"], best_fp32["persistent"])
+        for _ in range(3):
+            fr()
+        fp32_ms = timed(stream, fr, args.steps)
+
+    # ---- CPU baseline: the oracle's ME-TCF SpMM port on the host cores
+    cpu = None
+    if not args.no_cpu_baseline:
+        cores = len(os.sched_getaffinity(0))
+        m_host = {"RowWindowOffset": A.row_window_offset.cpu().numpy(),
+                  "TCblocktileId": A.tcblocktile_id.cpu().numpy(),
+                  "TCblockoffset": A.tcblock_offset.cpu().numpy(),
+                  "SparseAToXindex": A.sparse_atox.cpu().numpy()}
+        tcv = A.values_tc.cpu().numpy()
+        Cc = np.empty((M, N), np.float32)
+        oracle.spmm_metcf_f32(m_host, Bn, M, tc_values=tcv, nthreads=cores, out=Cc)
+        runs = []
+        for _ in range(3):
+            t = time.perf_counter()
+            oracle.spmm_metcf_f32(m_host, Bn, M, tc_values=tcv, nthreads=cores, out=Cc)
+            runs.append(time.perf_counter() - t)
+        t_cpu = float(np.median(runs))
+        tb = []
+        for _ in range(2):
+            t = time.perf_counter()
+            oracle.metcf(a.rowptr, a.col, M, values=a.values, nthreads=cores)
+            tb.append(time.perf_counter() - t)
+        cpu = {"value": flops / t_cpu / 1e9, "unit": "GFLOP/s", "cores": cores, "kind": "port",
+               "sample": f"full SpMM of this workload (R-MAT scale {args.scale}, N={N}) with the CPU "
+                         f"oracle's float32 ME-TCF port, median of 3 runs; host ME-TCF build "
+                         f"{np.median(tb) * 1e3:.0f} ms",
+               "cpu_model": platform.processor() or _cpu_model()}
+
+    result = {
+        "metric": "SpMM GFLOP/s (2*nnz*N) + % HBM roofline, mixed vs FP32 at <=1e-5 rel err",
+        "value": round(gflops, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp32": "f32", "bf16x6": "bf16x6(f32 acc)", "bf16x3": "bf16x3(f32 acc)",
+                  "fp16": "f16(f32 acc)", "bf16": "bf16(f32 acc)"}[chosen["precision"]],
+        "data": "synthetic (seeded R-MAT graph, B from torch N(0,1) seed 2)",
+        "config": {"workload": f"R-MAT scale {args.scale} (a=.57,b=.19,c=.19), edge factor 16, "
+                               f"symmetrised, 1/sqrt(deg_i deg_j) values; B {M}x{N} f32",
+                   "num_rows": M, "nnz": nnz, "N": N, "tc_blocks": T, "row_windows": W,
+                   "sum_unique_columns": sum_u,
+                   "variant": {k: chosen[k] for k in ("schedule", "precision", "chunk", "persistent")},
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "cpu_baseline": cpu,
+        "error_vs_fp64": {"componentwise": comp_err, "normwise": norm_err, "target": ERR_TARGET},
+        "fp32_path": None if fp32_ms is None else {
+            "ms": round(fp32_ms, 4), "gflops": round(flops / fp32_ms / 1e6, 2),
+            "variant": {k: best_fp32[k] for k in ("schedule", "chunk", "persistent")},
+            "speedup_of_chosen": round(fp32_ms / ms, 3)},
+        "preprocess_ms": round(float(np.median(pre_ms)), 3),
+        "bytes_model_per_launch": bytes_model,
+        "compulsory_bytes": compulsory_bytes(M, M, N, nnz),
+        "variants": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()} for r in table],
+    }
+    print(json.dumps(result), flush=True)
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,492 @@
+// The `DTCSpMM` Python module: the reference's operator surface
+// (DTC-SpMM/DTCSpMM.cpp:361-399) over the libdtc C ABI (include/dtc.h).
+//
+// Tensors are validated here (device, contiguity, dtype, sizes) before any
+// GPU work, device buffers come from torch's caching allocator, and every
+// launch goes to the current HIP stream of the input's device.  Library
+// failures raise RuntimeError (argument checks, as the reference's
+// TORCH_CHECKs) or ValueError (unknown plan or precision names), never exit
+// the process.
+#include <torch/extension.h>
+
+#include <ATen/hip/impl/HIPStreamMasqueradingAsCUDA.h>
+#include <hip/hip_runtime_api.h>
+
+#include <cstdlib>
+#include <fstream>
+#include <string>
+
+#include "dtc.h"
+
+namespace py = pybind11;
+using torch::Tensor;
+
+namespace {
+
+constexpr int kBlkH = 16;
+constexpr int kBlkW = 8;
+constexpr int kExeTime = 1000;  // timed launches under DTC_BENCH=1 (DTCSpMM_kernel.cu:18)
+
+#define CHECK_CUDA(x) TORCH_CHECK((x).is_cuda(), #x " must be a CUDA tensor")
+#define CHECK_CONTIGUOUS(x) TORCH_CHECK((x).is_contiguous(), #x " must be contiguous")
+#define CHECK_INPUT(x) \
+  CHECK_CUDA(x);       \
+  CHECK_CONTIGUOUS(x)
+#define CHECK_DTYPE(x, dt) \
+  TORCH_CHECK((x).scalar_type() == (dt), #x " must have dtype " #dt ", got ", (x).scalar_type())
+#define CHECK_SAME_DEVICE(x, ref) \
+  TORCH_CHECK((x).device() == (ref).device(), #x " must be on the same device as " #ref)
+
+dtc_stream_t current_stream() {
+  return reinterpret_cast<dtc_stream_t>(c10::hip::getCurrentHIPStreamMasqueradingAsCUDA().stream());
+}
+
+void check_status(dtc_status_t st, const char *where) {
+  if (st == DTC_SUCCESS) return;
+  const std::string msg = std::string(where) + ": " + dtc_status_string(st) + " (" + dtc_last_error() + ")";
+  TORCH_CHECK(false, msg);
+}
+
+bool env_flag(const char *name) {
+  const char *v = std::getenv(name);
+  return v && v[0] && std::string(v) != "0";
+}
+
+int32_t parse_precision(const std::string &p) {
+  if (p == "fp32" || p == "float32") return DTC_PRECISION_FP32;
+  if (p == "bf16" || p == "bfloat16") return DTC_PRECISION_BF16;
+  if (p == "fp16" || p == "float16") return DTC_PRECISION_FP16;
+  if (p == "bf16x3") return DTC_PRECISION_BF16X3;
+  if (p == "bf16x6") return DTC_PRECISION_BF16X6;
+  TORCH_CHECK_VALUE(false, "unknown precision '", p, "' (fp32, bf16, fp16, bf16x3, bf16x6)");
+  return 0;
+}
+
+// Reference exeplan names (DTCSpMM_kernel.cu:840-930) -> vector width and
+// column subtiles per wavefront.  The names only steer performance here:
+// every plan computes the full product for any N.
+void parse_exeplan(const std::string &plan, dtc_spmm_options_t &o) {
+  if (plan == "float_nonsplit" || plan == "float_split") { o.vec = 1; o.subtiles = 1; return; }
+  if (plan == "float2_nonsplit" || plan == "float2_split") { o.vec = 2; o.subtiles = 1; return; }
+  if (plan == "float4_nonsplit") { o.vec = 4; o.subtiles = 2; return; }
+  if (plan == "float4_split") { o.vec = 4; o.subtiles = 1; return; }
+  if (plan == "auto") { o.vec = 0; o.subtiles = 0; return; }
+  TORCH_CHECK_VALUE(false, "Not supported exe plan '", plan,
+                    "' (float_nonsplit, float2_nonsplit, float2_split, float4_nonsplit, float4_split, auto)");
+}
+
+// The widest vector width N and the base addresses allow, capped at `want`.
+int32_t fit_vec(int32_t want, int64_t n, const Tensor &b, const Tensor &c) {
+  for (int v : {4, 2, 1}) {
+    if (want != 0 && v > want) continue;
+    const bool ok = n % v == 0 && reinterpret_cast<uintptr_t>(b.data_ptr()) % (4 * v) == 0 &&
+                    reinterpret_cast<uintptr_t>(c.data_ptr()) % (4 * v) == 0;
+    if (ok) return v;
+  }
+  return 1;
+}
+
+struct Operands {
+  Tensor input, tile, off, atox;
+  c10::optional<Tensor> values;
+};
+
+void check_operands(const Operands &o, int64_t num_nodes) {
+  CHECK_INPUT(o.input);
+  CHECK_INPUT(o.tile);
+  CHECK_INPUT(o.off);
+  CHECK_INPUT(o.atox);
+  CHECK_DTYPE(o.input, torch::kFloat32);
+  CHECK_DTYPE(o.tile, torch::kUInt8);
+  CHECK_DTYPE(o.off, torch::kInt32);
+  CHECK_DTYPE(o.atox, torch::kInt32);
+  CHECK_SAME_DEVICE(o.tile, o.input);
+  CHECK_SAME_DEVICE(o.off, o.input);
+  CHECK_SAME_DEVICE(o.atox, o.input);
+  TORCH_CHECK(o.input.dim() == 2, "input must be 2-D [rows, N]");
+  TORCH_CHECK(o.input.size(1) >= 1, "input must have at least one column");
+  TORCH_CHECK(num_nodes >= 0, "num_nodes must be >= 0");
+  TORCH_CHECK(o.off.dim() == 1 && o.off.size(0) >= 1, "TCblock_offset must be 1-D with >= 1 entry");
+  const int64_t T = o.off.size(0) - 1;
+  TORCH_CHECK(o.atox.numel() >= T * kBlkW, "sparse_AToX_idx must hold 8 entries per TC block");
+  if (o.values.has_value()) {
+    const Tensor &v = *o.values;
+    CHECK_INPUT(v);
+    CHECK_DTYPE(v, torch::kFloat32);
+    CHECK_SAME_DEVICE(v, o.input);
+    TORCH_CHECK(v.numel() == o.tile.numel(), "values must hold one value per TCblocktile_id entry");
+  }
+}
+
+// Reference-named entries return a tensor shaped like input, as
+// zeros_like(input) does (DTCSpMM_kernel.cu:823), with rows the kernels do not
+// cover left at zero.  The general entry returns [num_nodes, N] or writes the
+// first num_nodes rows of `out`.
+Tensor output_for(const Tensor &input, int64_t num_nodes, bool ref_shape, const c10::optional<Tensor> &out,
+                  int64_t *m_eff) {
+  if (out.has_value()) {
+    const Tensor &c = *out;
+    CHECK_INPUT(c);
+    CHECK_DTYPE(c, torch::kFloat32);
+    CHECK_SAME_DEVICE(c, input);
+    TORCH_CHECK(c.dim() == 2 && c.size(1) == input.size(1), "out must be 2-D with N columns");
+    *m_eff = std::min<int64_t>(num_nodes, c.size(0));
+    return c;
+  }
+  if (!ref_shape) {
+    *m_eff = num_nodes;
+    return torch::empty({num_nodes, input.size(1)}, input.options());
+  }
+  const int64_t rows = input.size(0);
+  *m_eff = std::min<int64_t>(num_nodes, rows);
+  if (*m_eff == rows) return torch::empty_like(input);
+  return torch::zeros_like(input);
+}
+
+template <class Launch>
+void maybe_bench(const char *tag, int64_t num_edges, int64_t n, Launch launch) {
+  // DTC_BENCH=1 reproduces the reference's timed loop and CSV line
+  // (DTCSpMM_kernel.cu:840-937): 1000 launches, mean time, appended as
+  // "DTCSpMM:,<ms>,<GFLOP/s>," to ./DTCSpMM_exe_time_and_throughput.csv.
+  if (!env_flag("DTC_BENCH")) {
+    launch();
+    return;
+  }
+  hipStream_t s = reinterpret_cast<hipStream_t>(current_stream());
+  hipEvent_t e0, e1;
+  TORCH_CHECK(hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess, "hipEventCreate failed");
+  TORCH_CHECK(hipEventRecord(e0, s) == hipSuccess, "hipEventRecord failed");
+  for (int i = 0; i < kExeTime; ++i) launch();
+  TORCH_CHECK(hipEventRecord(e1, s) == hipSuccess, "hipEventRecord failed");
+  TORCH_CHECK(hipEventSynchronize(e1) == hipSuccess, "hipEventSynchronize failed");
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  const double t = ms / kExeTime;
+  const double gflops = 2.0 * (double)num_edges * (double)n / (t * 1e6);
+  std::ofstream f("DTCSpMM_exe_time_and_throughput.csv", std::ios::app);
+  f << "DTCSpMM:" << "," << t << "," << gflops << ",";
+  (void)tag;
+}
+
+// ------------------------------------------------------------ preprocess
+
+py::tuple preprocess_gpu(Tensor edgeList, Tensor nodePointer, int64_t num_nodes, int64_t blockSize_h,
+                         int64_t blockSize_w, Tensor blockPartition, Tensor edgeToColumn,
+                         Tensor edgeToRow, c10::optional<Tensor> values, bool return_edge_order) {
+  CHECK_INPUT(edgeList);
+  CHECK_INPUT(nodePointer);
+  CHECK_INPUT(blockPartition);
+  CHECK_INPUT(edgeToColumn);
+  CHECK_INPUT(edgeToRow);
+  for (const Tensor *t : {&edgeList, &nodePointer, &blockPartition, &edgeToColumn, &edgeToRow})
+    CHECK_DTYPE(*t, torch::kInt32);
+  CHECK_SAME_DEVICE(nodePointer, edgeList);
+  CHECK_SAME_DEVICE(blockPartition, edgeList);
+  CHECK_SAME_DEVICE(edgeToColumn, edgeList);
+  CHECK_SAME_DEVICE(edgeToRow, edgeList);
+  TORCH_CHECK(num_nodes >= 0 && num_nodes < INT32_MAX, "num_nodes out of range");
+  TORCH_CHECK(blockSize_h >= 1 && blockSize_w >= 1, "block sizes must be >= 1");
+  const int64_t nnz = edgeList.numel();
+  TORCH_CHECK(nnz < INT32_MAX, "more than 2^31-1 edges");
+  const int64_t W = (num_nodes + blockSize_h - 1) / blockSize_h;
+  TORCH_CHECK(nodePointer.numel() == num_nodes + 1, "nodePointer must have num_nodes + 1 entries, got ",
+              nodePointer.numel());
+  TORCH_CHECK(blockPartition.numel() == W, "blockPartition must have ceil(num_nodes / blockSize_h) = ", W,
+              " entries, got ", blockPartition.numel());
+  TORCH_CHECK(edgeToColumn.numel() == nnz, "edgeToColumn must have one entry per edge");
+  TORCH_CHECK(edgeToRow.numel() == nnz, "edgeToRow must have one entry per edge");
+  if (values.has_value()) {
+    CHECK_INPUT(*values);
+    CHECK_DTYPE(*values, torch::kFloat32);
+    CHECK_SAME_DEVICE(*values, edgeList);
+    TORCH_CHECK(values->numel() == nnz, "values must have one entry per edge");
+  }
+  const c10::OptionalDeviceGuard guard(edgeList.device());
+  const auto i32 = torch::TensorOptions().dtype(torch::kInt32).device(edgeList.device());
+  const auto u8 = torch::TensorOptions().dtype(torch::kUInt8).device(edgeList.device());
+  const auto f32 = torch::TensorOptions().dtype(torch::kFloat32).device(edgeList.device());
+  size_t ws_bytes = 0;
+  check_status(dtc_metcf_workspace_size((int32_t)num_nodes, (int32_t)nnz, (int32_t)blockSize_h,
+                                        (int32_t)blockSize_w, &ws_bytes),
+               "preprocess_gpu");
+  Tensor ws = torch::empty({(int64_t)ws_bytes}, u8);
+  Tensor rwo = torch::empty({W + 1}, i32);
+  dtc_stream_t s = current_stream();
+  int32_t T = 0;
+  check_status(dtc_metcf_build_windows(edgeList.data_ptr<int32_t>(), nodePointer.data_ptr<int32_t>(),
+                                       (int32_t)num_nodes, (int32_t)nnz, (int32_t)blockSize_h,
+                                       (int32_t)blockSize_w, blockPartition.data_ptr<int32_t>(),
+                                       edgeToColumn.data_ptr<int32_t>(), edgeToRow.data_ptr<int32_t>(),
+                                       rwo.data_ptr<int32_t>(), &T, ws.data_ptr(), ws_bytes, s),
+               "preprocess_gpu");
+  Tensor rowid = torch::empty({(int64_t)T}, i32);
+  Tensor tile = torch::empty({nnz}, u8);
+  Tensor off = torch::empty({(int64_t)T + 1}, i32);
+  Tensor atox = torch::empty({(int64_t)T * blockSize_w}, i32);
+  Tensor tcv, tce;
+  if (values.has_value()) tcv = torch::empty({nnz}, f32);
+  if (return_edge_order) tce = torch::empty({nnz}, i32);
+  check_status(dtc_metcf_build_blocks(nodePointer.data_ptr<int32_t>(), edgeToRow.data_ptr<int32_t>(),
+                                      rwo.data_ptr<int32_t>(), (int32_t)num_nodes, (int32_t)nnz,
+                                      (int32_t)blockSize_h, (int32_t)blockSize_w, T,
+                                      values.has_value() ? values->data_ptr<float>() : nullptr,
+                                      rowid.data_ptr<int32_t>(), tile.data_ptr<uint8_t>(),
+                                      off.data_ptr<int32_t>(), atox.data_ptr<int32_t>(),
+                                      values.has_value() ? tcv.data_ptr<float>() : nullptr,
+                                      return_edge_order ? tce.data_ptr<int32_t>() : nullptr,
+                                      ws.data_ptr(), ws_bytes, s),
+               "preprocess_gpu");
+  if (env_flag("DTC_VERBOSE"))
+    std::printf("TC_Blocks:\t%d\nExp_Edges:\t%lld\n", T, (long long)T * blockSize_w * blockSize_h);
+  py::list out;
+  out.append(rwo);
+  out.append(rowid);
+  out.append(tile);
+  out.append(off);
+  out.append(atox);
+  out.append(py::int_(T));
+  if (values.has_value()) out.append(tcv);
+  if (return_edge_order) out.append(tce);
+  return py::tuple(out);
+}
+
+// --------------------------------------------------------------- SpMM
+
+std::vector<Tensor> run_windows(const Operands &o, const Tensor &rwo, int64_t num_nodes, int64_t num_edges,
+                                dtc_spmm_options_t opt, bool bench, bool ref_shape, c10::optional<Tensor> out) {
+  CHECK_INPUT(rwo);
+  CHECK_DTYPE(rwo, torch::kInt32);
+  CHECK_SAME_DEVICE(rwo, o.input);
+  check_operands(o, num_nodes);
+  const int64_t W = rwo.numel() - 1;
+  TORCH_CHECK(W >= 0, "Rowwindow_offset must have >= 1 entry");
+  TORCH_CHECK(W * kBlkH >= num_nodes, "Rowwindow_offset covers ", W * kBlkH, " rows, fewer than num_nodes");
+  const c10::OptionalDeviceGuard guard(o.input.device());
+  int64_t m_eff = 0;
+  Tensor c = output_for(o.input, num_nodes, ref_shape, out, &m_eff);
+  const int64_t n = o.input.size(1);
+  opt.vec = fit_vec(opt.vec, n, o.input, c);
+  // The reference bounds B rows by num_nodes (dense_bound,
+  // DTCSpMM_kernel.cu:2687); the general entry takes B's own row count.
+  const int64_t b_rows = ref_shape ? std::min<int64_t>(num_nodes, o.input.size(0)) : o.input.size(0);
+  const float *vals = o.values.has_value() ? o.values->data_ptr<float>() : nullptr;
+  auto launch = [&]() {
+    check_status(dtc_spmm_windows(rwo.data_ptr<int32_t>(), o.tile.data_ptr<uint8_t>(), o.off.data_ptr<int32_t>(),
+                                  o.atox.data_ptr<int32_t>(), vals, (int32_t)W, (int32_t)(o.off.numel() - 1),
+                                  (int32_t)o.tile.numel(), (int32_t)m_eff,
+                                  o.input.data_ptr<float>(), (int32_t)b_rows, (int32_t)n, c.data_ptr<float>(),
+                                  &opt, current_stream()),
+                 "run_DTCSpMM");
+  };
+  if (bench) maybe_bench("DTCSpMM", num_edges, n, launch);
+  else launch();
+  return {c};
+}
+
+std::vector<Tensor> run_balanced(const Operands &o, const Tensor &rowid, int64_t num_nodes, dtc_spmm_options_t opt,
+                                 bool bench, bool ref_shape, c10::optional<Tensor> out) {
+  CHECK_INPUT(rowid);
+  CHECK_DTYPE(rowid, torch::kInt32);
+  CHECK_SAME_DEVICE(rowid, o.input);
+  check_operands(o, num_nodes);
+  const int64_t T = rowid.numel();
+  TORCH_CHECK(o.off.numel() == T + 1, "TCblock_offset must have one entry more than TCblock_rowid");
+  const c10::OptionalDeviceGuard guard(o.input.device());
+  int64_t m_eff = 0;
+  Tensor c = output_for(o.input, num_nodes, ref_shape, out, &m_eff);
+  const int64_t n = o.input.size(1);
+  opt.vec = fit_vec(opt.vec, n, o.input, c);
+  // The reference bounds B rows by num_nodes (dense_bound,
+  // DTCSpMM_kernel.cu:2687); the general entry takes B's own row count.
+  const int64_t b_rows = ref_shape ? std::min<int64_t>(num_nodes, o.input.size(0)) : o.input.size(0);
+  size_t ws_bytes = 0;
+  check_status(dtc_spmm_balanced_workspace_size((int32_t)T, &ws_bytes), "run_DTCSpMM_balance");
+  Tensor ws = torch::empty({(int64_t)ws_bytes}, o.input.options().dtype(torch::kUInt8));
+  const float *vals = o.values.has_value() ? o.values->data_ptr<float>() : nullptr;
+  auto launch = [&]() {
+    check_status(dtc_spmm_balanced(rowid.data_ptr<int32_t>(), o.tile.data_ptr<uint8_t>(), o.off.data_ptr<int32_t>(),
+                                   o.atox.data_ptr<int32_t>(), vals, (int32_t)T, (int32_t)o.tile.numel(),
+                                   (int32_t)m_eff,
+                                   o.input.data_ptr<float>(), (int32_t)b_rows, (int32_t)n, c.data_ptr<float>(), &opt,
+                                   ws.data_ptr(), ws_bytes, current_stream()),
+                 "run_DTCSpMM_balance");
+  };
+  if (bench) maybe_bench("DTCSpMM", o.tile.numel(), n, launch);
+  else launch();
+  return {c};
+}
+
+// run_DTCSpMM(input, Rowwindow_offset, TCblocktile_id, TCblock_offset,
+// sparse_AToX_idx, num_nodes, num_edges, exeplan)  -- DTCSpMM.cpp:117-131
+std::vector<Tensor> run_DTCSpMM(Tensor input, Tensor Rowwindow_offset, Tensor TCblocktile_id, Tensor TCblock_offset,
+                                Tensor sparse_AToX_idx, int64_t num_nodes, int64_t num_edges, std::string exeplan,
+                                c10::optional<Tensor> values, std::string precision) {
+  dtc_spmm_options_t opt{};
+  parse_exeplan(exeplan, opt);
+  opt.precision = parse_precision(precision);
+  Operands o{input, TCblocktile_id, TCblock_offset, sparse_AToX_idx, values};
+  return run_windows(o, Rowwindow_offset, num_nodes, num_edges, opt, true, true, c10::nullopt);
+}
+
+// run_DTCSpMM_balance(input, TCblock_rowid, TCblocktile_id, TCblock_offset,
+// sparse_AToX_idx, num_nodes, exeplan)  -- DTCSpMM.cpp:163-178
+std::vector<Tensor> run_DTCSpMM_balance(Tensor input, Tensor TCblock_rowid, Tensor TCblocktile_id,
+                                        Tensor TCblock_offset, Tensor sparse_AToX_idx, int64_t num_nodes,
+                                        std::string exeplan, c10::optional<Tensor> values, std::string precision,
+                                        int64_t chunk_blocks, bool persistent) {
+  dtc_spmm_options_t opt{};
+  parse_exeplan(exeplan, opt);
+  opt.precision = parse_precision(precision);
+  opt.chunk_blocks = (int32_t)chunk_blocks;
+  opt.persistent = persistent ? 1 : 0;
+  Operands o{input, TCblocktile_id, TCblock_offset, sparse_AToX_idx, values};
+  return run_balanced(o, TCblock_rowid, num_nodes, opt, true, true, c10::nullopt);
+}
+
+// DTCSpMM_gcn(input, Rowwindow_offset, TCblocktile_id, TCblock_offset,
+// sparse_AToX_idx, num_nodes, num_edges)  -- DTCSpMM.cpp:147-161: one launch,
+// variant picked from N as DTCSpMM_kernel.cu:1044-1108 does.
+std::vector<Tensor> DTCSpMM_gcn(Tensor input, Tensor Rowwindow_offset, Tensor TCblocktile_id, Tensor TCblock_offset,
+                                Tensor sparse_AToX_idx, int64_t num_nodes, int64_t num_edges,
+                                c10::optional<Tensor> values) {
+  dtc_spmm_options_t opt{};
+  const int64_t n = input.dim() == 2 ? input.size(1) : 0;
+  if (n >= 128 && n % 4 == 0) { opt.vec = 4; opt.subtiles = 1; }
+  else if (n % 2 == 0) { opt.vec = 2; opt.subtiles = 1; }
+  else { opt.vec = 1; opt.subtiles = 1; }
+  opt.precision = DTC_PRECISION_FP32;
+  Operands o{input, TCblocktile_id, TCblock_offset, sparse_AToX_idx, values};
+  return run_windows(o, Rowwindow_offset, num_nodes, num_edges, opt, false, true, c10::nullopt);
+}
+
+// General entry (build extension): either schedule, every tuning knob, an
+// optional preallocated output, no CSV side effects.
+Tensor spmm(Tensor input, Tensor window_or_rowid, Tensor TCblocktile_id, Tensor TCblock_offset, Tensor sparse_AToX_idx,
+            int64_t num_nodes, std::string schedule, c10::optional<Tensor> values, std::string precision, int64_t vec,
+            int64_t subtiles, int64_t chunk_blocks, bool persistent, c10::optional<Tensor> out) {
+  dtc_spmm_options_t opt{};
+  opt.precision = parse_precision(precision);
+  TORCH_CHECK_VALUE(vec == 0 || vec == 1 || vec == 2 || vec == 4, "vec must be 0, 1, 2 or 4");
+  TORCH_CHECK_VALUE(subtiles >= 0 && subtiles <= 2, "subtiles must be 0, 1 or 2");
+  opt.vec = (int32_t)vec;
+  opt.subtiles = (int32_t)subtiles;
+  opt.chunk_blocks = (int32_t)chunk_blocks;
+  opt.persistent = persistent ? 1 : 0;
+  Operands o{input, TCblocktile_id, TCblock_offset, sparse_AToX_idx, values};
+  if (schedule == "windows" || schedule == "unbalanced")
+    return run_windows(o, window_or_rowid, num_nodes, TCblocktile_id.numel(), opt, false, false, out)[0];
+  if (schedule == "balanced")
+    return run_balanced(o, window_or_rowid, num_nodes, opt, false, false, out)[0];
+  TORCH_CHECK_VALUE(false, "schedule must be 'windows' or 'balanced', got '", schedule, "'");
+  return Tensor();
+}
+
+// ------------------------------------------------------------- backward
+
+py::tuple csr_transpose(Tensor nodePointer, Tensor edgeList, int64_t num_rows, int64_t num_cols) {
+  CHECK_INPUT(nodePointer);
+  CHECK_INPUT(edgeList);
+  CHECK_DTYPE(nodePointer, torch::kInt32);
+  CHECK_DTYPE(edgeList, torch::kInt32);
+  CHECK_SAME_DEVICE(edgeList, nodePointer);
+  TORCH_CHECK(num_rows >= 0 && num_cols >= 0, "sizes must be >= 0");
+  TORCH_CHECK(nodePointer.numel() == num_rows + 1, "nodePointer must have num_rows + 1 entries");
+  const int64_t nnz = edgeList.numel();
+  const c10::OptionalDeviceGuard guard(nodePointer.device());
+  const auto i32 = nodePointer.options();
+  size_t ws_bytes = 0;
+  check_status(dtc_csr_transpose_workspace_size((int32_t)num_rows, (int32_t)num_cols, (int32_t)nnz, &ws_bytes),
+               "csr_transpose");
+  Tensor ws = torch::empty({(int64_t)ws_bytes}, i32.dtype(torch::kUInt8));
+  Tensor rpt = torch::empty({num_cols + 1}, i32);
+  Tensor ct = torch::empty({nnz}, i32);
+  Tensor perm = torch::empty({nnz}, i32);
+  check_status(dtc_csr_transpose(nodePointer.data_ptr<int32_t>(), edgeList.data_ptr<int32_t>(), (int32_t)num_rows,
+                                 (int32_t)num_cols, (int32_t)nnz, rpt.data_ptr<int32_t>(), ct.data_ptr<int32_t>(),
+                                 perm.data_ptr<int32_t>(), ws.data_ptr(), ws_bytes, current_stream()),
+               "csr_transpose");
+  return py::make_tuple(rpt, ct, perm);
+}
+
+Tensor sddmm_csr(Tensor nodePointer, Tensor edgeList, Tensor X, Tensor Y) {
+  CHECK_INPUT(nodePointer);
+  CHECK_INPUT(edgeList);
+  CHECK_INPUT(X);
+  CHECK_INPUT(Y);
+  CHECK_DTYPE(nodePointer, torch::kInt32);
+  CHECK_DTYPE(edgeList, torch::kInt32);
+  CHECK_DTYPE(X, torch::kFloat32);
+  CHECK_DTYPE(Y, torch::kFloat32);
+  TORCH_CHECK(X.dim() == 2 && Y.dim() == 2 && X.size(1) == Y.size(1), "X and Y must be 2-D with equal widths");
+  TORCH_CHECK(nodePointer.numel() == X.size(0) + 1, "X must have one row per CSR row");
+  const c10::OptionalDeviceGuard guard(X.device());
+  Tensor out = torch::empty({edgeList.numel()}, X.options());
+  check_status(dtc_sddmm_csr(nodePointer.data_ptr<int32_t>(), edgeList.data_ptr<int32_t>(), (int32_t)X.size(0),
+                             (int32_t)edgeList.numel(), X.data_ptr<float>(), Y.data_ptr<float>(), (int32_t)Y.size(0),
+                             (int32_t)X.size(1), out.data_ptr<float>(), current_stream()),
+               "sddmm_csr");
+  return out;
+}
+
+Tensor gather(Tensor values, Tensor index) {
+  CHECK_INPUT(values);
+  CHECK_INPUT(index);
+  CHECK_DTYPE(values, torch::kFloat32);
+  CHECK_DTYPE(index, torch::kInt32);
+  const c10::OptionalDeviceGuard guard(values.device());
+  Tensor out = torch::empty({index.numel()}, values.options());
+  check_status(dtc_gather_f32(values.data_ptr<float>(), index.data_ptr<int32_t>(), (int32_t)index.numel(),
+                              out.data_ptr<float>(), current_stream()),
+               "gather");
+  return out;
+}
+
+}  // namespace
+
+PYBIND11_MODULE(DTCSpMM, m) {
+  m.doc() = "DTC-SpMM for AMD MI355X (gfx950): ME-TCF build and SpMM over libdtc";
+  m.def("preprocess_gpu", &preprocess_gpu, "ME-TCF build on the GPU", py::arg("edgeList"), py::arg("nodePointer"),
+        py::arg("num_nodes"), py::arg("blockSize_h"), py::arg("blockSize_w"), py::arg("blockPartition"),
+        py::arg("edgeToColumn"), py::arg("edgeToRow"), py::arg("values") = py::none(),
+        py::arg("return_edge_order") = false);
+  m.def("run_DTCSpMM", &run_DTCSpMM, "DTC-SpMM forward, one work unit per row window", py::arg("input"),
+        py::arg("Rowwindow_offset"), py::arg("TCblocktile_id"), py::arg("TCblock_offset"), py::arg("sparse_AToX_idx"),
+        py::arg("num_nodes"), py::arg("num_edges"), py::arg("exeplan"), py::arg("values") = py::none(),
+        py::arg("precision") = "fp32");
+  m.def("run_DTCSpMM_balance", &run_DTCSpMM_balance, "DTC-SpMM forward, balanced TC-block chunks", py::arg("input"),
+        py::arg("TCblock_rowid"), py::arg("TCblocktile_id"), py::arg("TCblock_offset"), py::arg("sparse_AToX_idx"),
+        py::arg("num_nodes"), py::arg("exeplan"), py::arg("values") = py::none(), py::arg("precision") = "fp32",
+        py::arg("chunk_blocks") = 0, py::arg("persistent") = false);
+  m.def("DTCSpMM_gcn", &DTCSpMM_gcn, "DTC-SpMM forward, single launch, variant chosen from N", py::arg("input"),
+        py::arg("Rowwindow_offset"), py::arg("TCblocktile_id"), py::arg("TCblock_offset"), py::arg("sparse_AToX_idx"),
+        py::arg("num_nodes"), py::arg("num_edges"), py::arg("values") = py::none());
+  // The reference binds these names to the forward functions
+  // (DTCSpMM.cpp:386-393); kept as the same aliases.  True gradients are in
+  // the dtc.autograd module.
+  m.def("backward_ptx_uint8_improved", &run_DTCSpMM, "alias of run_DTCSpMM (reference behaviour)", py::arg("input"),
+        py::arg("Rowwindow_offset"), py::arg("TCblocktile_id"), py::arg("TCblock_offset"), py::arg("sparse_AToX_idx"),
+        py::arg("num_nodes"), py::arg("num_edges"), py::arg("exeplan"), py::arg("values") = py::none(),
+        py::arg("precision") = "fp32");
+  m.def("backward_ptx_uint8_improved_gcn", &DTCSpMM_gcn, "alias of DTCSpMM_gcn (reference behaviour)",
+        py::arg("input"), py::arg("Rowwindow_offset"), py::arg("TCblocktile_id"), py::arg("TCblock_offset"),
+        py::arg("sparse_AToX_idx"), py::arg("num_nodes"), py::arg("num_edges"), py::arg("values") = py::none());
+  m.def("balance_backward_ptx_uint8_prefetch", &run_DTCSpMM_balance, "alias of run_DTCSpMM_balance",
+        py::arg("input"), py::arg("TCblock_rowid"), py::arg("TCblocktile_id"), py::arg("TCblock_offset"),
+        py::arg("sparse_AToX_idx"), py::arg("num_nodes"), py::arg("exeplan"), py::arg("values") = py::none(),
+        py::arg("precision") = "fp32", py::arg("chunk_blocks") = 0, py::arg("persistent") = false);
+  m.def("spmm", &spmm, "SpMM with explicit schedule and tuning knobs", py::arg("input"), py::arg("window_or_rowid"),
+        py::arg("TCblocktile_id"), py::arg("TCblock_offset"), py::arg("sparse_AToX_idx"), py::arg("num_nodes"),
+        py::arg("schedule") = "balanced", py::arg("values") = py::none(), py::arg("precision") = "fp32",
+        py::arg("vec") = 0, py::arg("subtiles") = 0, py::arg("chunk_blocks") = 0, py::arg("persistent") = false,
+        py::arg("out") = py::none());
+  m.def("csr_transpose", &csr_transpose, "CSR transpose on the GPU -> (rowptr_t, col_t, perm)",
+        py::arg("nodePointer"), py::arg("edgeList"), py::arg("num_rows"), py::arg("num_cols"));
+  m.def("sddmm_csr", &sddmm_csr, "out[e] = X[row(e)] . Y[col(e)]", py::arg("nodePointer"), py::arg("edgeList"),
+        py::arg("X"), py::arg("Y"));
+  m.def("gather", &gather, "out[p] = values[index[p]]", py::arg("values"), py::arg("index"));
+  m.attr("BLK_H") = kBlkH;
+  m.attr("BLK_W") = kBlkW;
+  m.attr("library_version") = dtc_version();
+}

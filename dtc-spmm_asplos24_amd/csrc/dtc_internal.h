@@ -1,0 +1,146 @@
+// Internal helpers shared by the libdtc HIP sources (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "dtc.h"
+
+namespace dtc {
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------- status
+
+// Records `msg` as the calling thread's last error detail and returns `st`.
+dtc_status_t fail(dtc_status_t st, const std::string &msg);
+void clear_error();
+
+#define DTC_HIP_CHECK(expr)                                                        \
+  do {                                                                             \
+    hipError_t _e = (expr);                                                        \
+    if (_e != hipSuccess)                                                          \
+      return ::dtc::fail(DTC_ERROR_HIP, std::string(#expr " failed: ") +           \
+                                            hipGetErrorString(_e));                \
+  } while (0)
+
+#define DTC_LAUNCH_CHECK(what)                                                     \
+  do {                                                                             \
+    hipError_t _e = hipGetLastError();                                             \
+    if (_e != hipSuccess)                                                          \
+      return ::dtc::fail(DTC_ERROR_HIP, std::string(what " launch failed: ") +     \
+                                            hipGetErrorString(_e));                \
+  } while (0)
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Carves aligned sub-buffers out of a caller workspace.
+struct Carver {
+  char *base;
+  size_t off = 0;
+  explicit Carver(void *p) : base(static_cast<char *>(p)) {}
+  template <class T>
+  T *take(size_t count) {
+    off = align_up(off, 256);
+    T *p = base ? reinterpret_cast<T *>(base + off) : nullptr;
+    off += count * sizeof(T);
+    return p;
+  }
+  size_t used() const { return align_up(off, 256); }
+};
+
+// ------------------------------------------------------ wave64 primitives
+
+__device__ __forceinline__ int lane_id() {
+  return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+// Orders LDS accesses of one wavefront (lanes exchange data through LDS
+// without a workgroup barrier).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int wave_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// Inclusive prefix sum across the 64 lanes of a wavefront.
+__device__ __forceinline__ int wave_inclusive_sum(int x) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const int t = __shfl_up(x, d, kWave);
+    if (lane >= d) x += t;
+  }
+  return x;
+}
+
+__device__ __forceinline__ int wave_sum(int x) {
+#pragma unroll
+  for (int d = kWave / 2; d >= 1; d >>= 1) x += __shfl_xor(x, d, kWave);
+  return x;
+}
+
+__device__ __forceinline__ int wave_max(int x) {
+#pragma unroll
+  for (int d = kWave / 2; d >= 1; d >>= 1) x = max(x, __shfl_xor(x, d, kWave));
+  return x;
+}
+
+// Inclusive prefix sum across a 256-thread workgroup; `total` receives the
+// workgroup sum.  `smem` needs 4 ints.  Contains __syncthreads().
+__device__ __forceinline__ int block256_inclusive_sum(int x, int *smem, int &total) {
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  int inc = wave_inclusive_sum(x);
+  if (lane == kWave - 1) smem[wave] = inc;
+  __syncthreads();
+  int prefix = 0;
+  total = 0;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    const int s = smem[w];
+    if (w < wave) prefix += s;
+    total += s;
+  }
+  __syncthreads();
+  return inc + prefix;
+}
+
+// First index in [lo, hi) with a[idx] >= key (a ascending).
+template <class T>
+__device__ __forceinline__ int lower_bound_dev(const T *a, int lo, int hi, T key) {
+  while (lo < hi) {
+    const int mid = lo + ((hi - lo) >> 1);
+    if (a[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// ----------------------------------------------------- device-wide pieces
+// (implemented in primitives.hip)
+
+// Workspace bytes for a scan over n int32 elements.
+size_t scan_workspace_bytes(int64_t n);
+// out[i] = sum_{j <= i} in[j] (inclusive) or sum_{j < i} (exclusive).
+// `in` and `out` may alias.
+dtc_status_t scan_i32(const int32_t *in, int32_t *out, int64_t n, bool inclusive,
+                      void *ws, hipStream_t stream);
+// Inclusive scan of (keys[i] != keys[i-1]) with keys[-1] treated as different.
+dtc_status_t scan_key_changes(const uint64_t *keys, int32_t *out, int64_t n, void *ws,
+                              hipStream_t stream);
+
+// Stable LSD radix sort of (key, value) pairs by bits [0, key_bits).
+// Result lands in (keys0, vals0) if `*result_in_1` is false, else in
+// (keys1, vals1).
+size_t radix_sort_workspace_bytes(int64_t n);
+dtc_status_t radix_sort_pairs(uint64_t *keys0, int32_t *vals0, uint64_t *keys1,
+                              int32_t *vals1, int64_t n, int key_bits, bool *result_in_1,
+                              void *ws, hipStream_t stream);
+
+}  // namespace dtc

@@ -1,0 +1,269 @@
+// Device-wide building blocks for the ME-TCF build on gfx950 (wave64):
+// reduce-then-scan prefix sums and a stable LSD radix sort of
+// (uint64 key, int32 value) pairs whose per-tile ranking uses wave ballots.
+#include <string>
+
+#include "dtc_internal.h"
+
+namespace dtc {
+
+// ------------------------------------------------------------------ status
+
+static thread_local std::string g_last_error;
+
+dtc_status_t fail(dtc_status_t st, const std::string &msg) {
+  g_last_error = msg;
+  return st;
+}
+void clear_error() { g_last_error.clear(); }
+
+// -------------------------------------------------------------------- scan
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+struct LoadI32 {
+  const int32_t *p;
+  __device__ __forceinline__ int operator()(int64_t i) const { return p[i]; }
+};
+
+struct LoadKeyChange {
+  const uint64_t *k;
+  __device__ __forceinline__ int operator()(int64_t i) const {
+    return (i == 0 || k[i] != k[i - 1]) ? 1 : 0;
+  }
+};
+
+template <class Load>
+__global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(Load load, int64_t n,
+                                                                    int32_t *partials) {
+  __shared__ int smem[4];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  int sum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t i = base + k * kScanThreads + threadIdx.x;
+    if (i < n) sum += load(i);
+  }
+  sum = wave_sum(sum);
+  if ((threadIdx.x & 63) == 0) smem[threadIdx.x >> 6] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) partials[blockIdx.x] = smem[0] + smem[1] + smem[2] + smem[3];
+}
+
+// Exclusive scan of partials[0..m) in place, one workgroup.
+__global__ __launch_bounds__(kScanThreads) void scan_partials_kernel(int32_t *partials,
+                                                                      int64_t m) {
+  __shared__ int smem[4];
+  int carry = 0;
+  for (int64_t base = 0; base < m; base += kScanTile) {
+    int v[kScanItems];
+    int local = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      const int64_t i = base + (int64_t)threadIdx.x * kScanItems + k;
+      v[k] = i < m ? partials[i] : 0;
+      local += v[k];
+    }
+    int total;
+    const int inc = block256_inclusive_sum(local, smem, total);
+    int run = carry + inc - local;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      const int64_t i = base + (int64_t)threadIdx.x * kScanItems + k;
+      if (i < m) partials[i] = run;
+      run += v[k];
+    }
+    carry += total;
+  }
+}
+
+template <class Load>
+__global__ __launch_bounds__(kScanThreads) void scan_downsweep_kernel(
+    Load load, int32_t *out, int64_t n, const int32_t *partials, int inclusive) {
+  __shared__ int smem[4];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  int v[kScanItems];
+  int local = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t i = base + k;
+    v[k] = i < n ? load(i) : 0;
+    local += v[k];
+  }
+  int total;
+  const int inc = block256_inclusive_sum(local, smem, total);
+  int run = partials[blockIdx.x] + inc - local;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t i = base + k;
+    if (inclusive) run += v[k];
+    if (i < n) out[i] = run;
+    if (!inclusive) run += v[k];
+  }
+}
+
+size_t scan_workspace_bytes(int64_t n) {
+  return align_up(sizeof(int32_t) * (size_t)std::max<int64_t>(1, ceil_div(n, kScanTile)), 256);
+}
+
+template <class Load>
+static dtc_status_t scan_impl(Load load, int32_t *out, int64_t n, bool inclusive, void *ws,
+                              hipStream_t stream) {
+  if (n <= 0) return DTC_SUCCESS;
+  const int64_t tiles = ceil_div(n, kScanTile);
+  if (tiles > INT32_MAX) return fail(DTC_ERROR_UNSUPPORTED, "scan too large");
+  int32_t *partials = static_cast<int32_t *>(ws);
+  scan_reduce_kernel<Load><<<(unsigned)tiles, kScanThreads, 0, stream>>>(load, n, partials);
+  DTC_LAUNCH_CHECK("scan_reduce");
+  scan_partials_kernel<<<1, kScanThreads, 0, stream>>>(partials, tiles);
+  DTC_LAUNCH_CHECK("scan_partials");
+  scan_downsweep_kernel<Load>
+      <<<(unsigned)tiles, kScanThreads, 0, stream>>>(load, out, n, partials, inclusive ? 1 : 0);
+  DTC_LAUNCH_CHECK("scan_downsweep");
+  return DTC_SUCCESS;
+}
+
+dtc_status_t scan_i32(const int32_t *in, int32_t *out, int64_t n, bool inclusive, void *ws,
+                      hipStream_t stream) {
+  return scan_impl(LoadI32{in}, out, n, inclusive, ws, stream);
+}
+
+dtc_status_t scan_key_changes(const uint64_t *keys, int32_t *out, int64_t n, void *ws,
+                              hipStream_t stream) {
+  return scan_impl(LoadKeyChange{keys}, out, n, true, ws, stream);
+}
+
+// -------------------------------------------------------------- radix sort
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kRadix = 256;
+
+// hist[d * ntiles + tile] = number of keys in the tile whose digit is d.
+__global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint64_t *keys,
+                                                                   int64_t n, int shift,
+                                                                   int32_t *hist,
+                                                                   int ntiles) {
+  __shared__ int cnt[kRadix];
+  cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kSortTile;
+#pragma unroll 4
+  for (int k = 0; k < kSortItems; ++k) {
+    const int64_t i = base + k * kSortThreads + threadIdx.x;
+    if (i < n) atomicAdd(&cnt[(int)((keys[i] >> shift) & (kRadix - 1))], 1);
+  }
+  __syncthreads();
+  hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Stable scatter: element i of the tile goes to offsets[d][tile] + (number of
+// earlier elements of the tile with the same digit d).  Elements are taken in
+// rounds of 256 in index order; inside a round the rank among equal digits is
+// found with eight ballots (one per digit bit) and a popcount.
+__global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
+    const uint64_t *kin, const int32_t *vin, uint64_t *kout, int32_t *vout, int64_t n,
+    int shift, const int32_t *offsets, int ntiles) {
+  __shared__ int s_off[kRadix];
+  __shared__ int s_wcnt[4][kRadix];
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+  const int lane = tid & 63;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  s_off[tid] = offsets[(int64_t)tid * ntiles + blockIdx.x];
+  const int64_t base = (int64_t)blockIdx.x * kSortTile;
+  for (int r = 0; r < kSortItems; ++r) {
+    s_wcnt[0][tid] = 0;
+    s_wcnt[1][tid] = 0;
+    s_wcnt[2][tid] = 0;
+    s_wcnt[3][tid] = 0;
+    __syncthreads();
+    const int64_t i = base + (int64_t)r * kSortThreads + tid;
+    const bool valid = i < n;
+    const uint64_t key = valid ? kin[i] : 0ull;
+    const int32_t val = valid ? vin[i] : 0;
+    const int d = (int)((key >> shift) & (kRadix - 1));
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < 8; ++bit) {
+      const bool b = (d >> bit) & 1;
+      const uint64_t m = __ballot(b);
+      peers &= b ? m : ~m;
+    }
+    const int rank = __popcll(peers & lt_mask);
+    if (valid && rank == 0) s_wcnt[wave][d] = __popcll(peers);
+    __syncthreads();
+    if (valid) {
+      int pre = 0;
+      for (int w = 0; w < wave; ++w) pre += s_wcnt[w][d];
+      const int64_t pos = (int64_t)s_off[d] + pre + rank;
+      kout[pos] = key;
+      vout[pos] = val;
+    }
+    __syncthreads();
+    s_off[tid] += s_wcnt[0][tid] + s_wcnt[1][tid] + s_wcnt[2][tid] + s_wcnt[3][tid];
+    __syncthreads();
+  }
+}
+
+size_t radix_sort_workspace_bytes(int64_t n) {
+  const int64_t ntiles = std::max<int64_t>(1, ceil_div(n, kSortTile));
+  return align_up(sizeof(int32_t) * (size_t)(ntiles * kRadix), 256) +
+         scan_workspace_bytes(ntiles * kRadix);
+}
+
+dtc_status_t radix_sort_pairs(uint64_t *keys0, int32_t *vals0, uint64_t *keys1,
+                              int32_t *vals1, int64_t n, int key_bits, bool *result_in_1,
+                              void *ws, hipStream_t stream) {
+  *result_in_1 = false;
+  if (n <= 1 || key_bits <= 0) return DTC_SUCCESS;
+  const int64_t ntiles = ceil_div(n, kSortTile);
+  if (ntiles * kRadix > INT32_MAX) return fail(DTC_ERROR_UNSUPPORTED, "sort too large");
+  Carver c(ws);
+  int32_t *hist = c.take<int32_t>((size_t)(ntiles * kRadix));
+  void *scan_ws = c.take<char>(scan_workspace_bytes(ntiles * kRadix));
+  uint64_t *kin = keys0, *kout = keys1;
+  int32_t *vin = vals0, *vout = vals1;
+  bool in_1 = false;
+  for (int shift = 0; shift < key_bits; shift += 8) {
+    radix_hist_kernel<<<(unsigned)ntiles, kSortThreads, 0, stream>>>(kin, n, shift, hist,
+                                                                      (int)ntiles);
+    DTC_LAUNCH_CHECK("radix_hist");
+    dtc_status_t st = scan_i32(hist, hist, ntiles * kRadix, false, scan_ws, stream);
+    if (st != DTC_SUCCESS) return st;
+    radix_scatter_kernel<<<(unsigned)ntiles, kSortThreads, 0, stream>>>(
+        kin, vin, kout, vout, n, shift, hist, (int)ntiles);
+    DTC_LAUNCH_CHECK("radix_scatter");
+    std::swap(kin, kout);
+    std::swap(vin, vout);
+    in_1 = !in_1;
+  }
+  *result_in_1 = in_1;
+  return DTC_SUCCESS;
+}
+
+}  // namespace dtc
+
+extern "C" {
+
+const char *dtc_status_string(dtc_status_t status) {
+  switch (status) {
+    case DTC_SUCCESS: return "success";
+    case DTC_ERROR_INVALID_VALUE: return "invalid value";
+    case DTC_ERROR_UNSUPPORTED: return "unsupported";
+    case DTC_ERROR_WORKSPACE: return "workspace too small";
+    case DTC_ERROR_INVALID_CSR: return "invalid CSR";
+    case DTC_ERROR_HIP: return "HIP error";
+    case DTC_ERROR_INCONSISTENT: return "inconsistent blockPartition";
+  }
+  return "unknown status";
+}
+
+const char *dtc_last_error(void) { return dtc::g_last_error.c_str(); }
+
+int32_t dtc_version(void) { return 10000; }
+
+}  // extern "C"

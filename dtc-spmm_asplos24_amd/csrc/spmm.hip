@@ -1,0 +1,617 @@
+// ME-TCF SpMM on gfx950: C = A * B with A in 16 x 8 TC blocks.
+//
+// Work decomposition (one wavefront = one work unit, 4 waves per workgroup):
+//   unit = (run of TC blocks of ONE row window, column tile of 16*VEC*NSUB
+//   output columns).  The unbalanced schedule uses a whole window per unit
+//   (DTCSpMM_kernel.cu:811-944 launches one CTA per window); the balanced
+//   schedule cuts the TC-block list into fixed chunks and splits every chunk
+//   at window boundaries (DTCSpMM_kernel.cu:4730-4901 walks 64-block chunks
+//   and flushes on a window change).
+//
+// Inner loop: 4 consecutive TC blocks of the window form one K = 32 group,
+// which is exactly the K of v_mfma_f32_16x16x32_{bf16,f16}.  Lane l works on
+// block g = l / 16 of the group and MFMA column q = l % 16:
+//   * B operand: the lane gathers the 8 rows SparseAToX[8*(bb+g) + j] of B,
+//     VEC consecutive floats each at column tile + q*VEC (one 16-lane group
+//     reads 16*VEC*4 contiguous bytes of a row);
+//   * A operand: the lane needs A_g[q][0..7], read from a per-wave LDS tile
+//     built by scattering the stored values at their tile ids;
+//   * C: lane l holds rows 4*(l/16) + i, i < 4, of its MFMA column, i.e. the
+//     output columns tile + q*VEC + v for the v-th MFMA.
+// For FP32 the same group is fed as 8 steps of v_mfma_f32_16x16x4_f32 where
+// step j pairs A_g[:, j] with B row j of block g.  The k order inside a
+// group is a permutation shared by A and B, so every mapping sums the same
+// products.
+#include <algorithm>
+#include <string>
+
+#include "dtc_internal.h"
+
+namespace dtc {
+namespace {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
+
+constexpr int kBlkH = 16;
+constexpr int kBlkW = 8;
+constexpr int kGroup = 4;                        // TC blocks per K=32 group
+constexpr int kTileFloats = kGroup * kBlkH * kBlkW;  // 512
+constexpr int kWavesPerBlock = 4;
+
+struct SpmmParams {
+  const int32_t *rwo;    // unbalanced: row window offsets [W + 1]
+  const int32_t *rowid;  // balanced: window of every TC block [T]
+  const uint8_t *tile;
+  const int32_t *off;
+  const int32_t *atox;
+  const float *vals;  // TC-block order, NULL = all ones
+  const float *B;
+  float *C;
+  int32_t num_windows;
+  int32_t num_blocks;
+  int32_t M;
+  int32_t b_rows;
+  int32_t N;
+  int32_t nnz;         // entries of tile / vals
+  int32_t n_ct;        // column tiles
+  int32_t num_items;   // windows (unbalanced) or chunks (balanced)
+  int32_t chunk;       // balanced chunk length in TC blocks
+  int32_t *queue;      // persistent work counter
+};
+
+template <int VEC>
+struct VecType {
+  typedef float type __attribute__((ext_vector_type(VEC)));
+};
+template <>
+struct VecType<1> {
+  typedef float type;
+};
+
+template <int VEC>
+__device__ __forceinline__ float comp(const typename VecType<VEC>::type &x, int v) {
+  if constexpr (VEC == 1) return x;
+  else return x[v];
+}
+
+template <int VEC>
+__device__ __forceinline__ typename VecType<VEC>::type vzero() {
+  typename VecType<VEC>::type z;
+  if constexpr (VEC == 1) z = 0.f;
+  else
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) z[v] = 0.f;
+  return z;
+}
+
+__device__ __forceinline__ float bf16_to_f32(__bf16 x) { return (float)x; }
+
+// Limb split of x into bf16 terms, x = l0 + l1 (+ l2) + residual.
+__device__ __forceinline__ void split2(float x, __bf16 &h, __bf16 &l) {
+  h = (__bf16)x;
+  l = (__bf16)(x - bf16_to_f32(h));
+}
+__device__ __forceinline__ void split3(float x, __bf16 &h, __bf16 &m, __bf16 &l) {
+  h = (__bf16)x;
+  const float r1 = x - bf16_to_f32(h);
+  m = (__bf16)r1;
+  l = (__bf16)(r1 - bf16_to_f32(m));
+}
+
+#define MFMA_F32(a, b, c) __builtin_amdgcn_mfma_f32_16x16x4f32((a), (b), (c), 0, 0, 0)
+#define MFMA_BF16(a, b, c) __builtin_amdgcn_mfma_f32_16x16x32_bf16((a), (b), (c), 0, 0, 0)
+#define MFMA_F16(a, b, c) __builtin_amdgcn_mfma_f32_16x16x32_f16((a), (b), (c), 0, 0, 0)
+
+// acc[s][v] += A_group * B_group for the columns of subtile s, component v.
+template <int PREC, int VEC, int NSUB>
+__device__ __forceinline__ void mfma_group(const float (&a)[8],
+                                           const typename VecType<VEC>::type (&breg)[NSUB][8],
+                                           floatx4 (&acc)[NSUB][VEC], bool has_vals) {
+  if constexpr (PREC == DTC_PRECISION_FP32) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+#pragma unroll
+      for (int s = 0; s < NSUB; ++s)
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) acc[s][v] = MFMA_F32(a[j], comp<VEC>(breg[s][j], v), acc[s][v]);
+  } else if constexpr (PREC == DTC_PRECISION_BF16) {
+    bf16x8 A;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) A[j] = (__bf16)a[j];
+#pragma unroll
+    for (int s = 0; s < NSUB; ++s)
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+        bf16x8 Bv;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) Bv[j] = (__bf16)comp<VEC>(breg[s][j], v);
+        acc[s][v] = MFMA_BF16(A, Bv, acc[s][v]);
+      }
+  } else if constexpr (PREC == DTC_PRECISION_FP16) {
+    halfx8 A;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) A[j] = (_Float16)a[j];
+#pragma unroll
+    for (int s = 0; s < NSUB; ++s)
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+        halfx8 Bv;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) Bv[j] = (_Float16)comp<VEC>(breg[s][j], v);
+        acc[s][v] = MFMA_F16(A, Bv, acc[s][v]);
+      }
+  } else if constexpr (PREC == DTC_PRECISION_BF16X3) {
+    bf16x8 Ah, Al;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      __bf16 h, l;
+      split2(a[j], h, l);
+      Ah[j] = h;
+      Al[j] = l;
+    }
+#pragma unroll
+    for (int s = 0; s < NSUB; ++s)
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+        bf16x8 Bh, Bl;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          __bf16 h, l;
+          split2(comp<VEC>(breg[s][j], v), h, l);
+          Bh[j] = h;
+          Bl[j] = l;
+        }
+        floatx4 c = acc[s][v];
+        if (has_vals) c = MFMA_BF16(Al, Bh, c);
+        c = MFMA_BF16(Ah, Bl, c);
+        c = MFMA_BF16(Ah, Bh, c);
+        acc[s][v] = c;
+      }
+  } else {  // DTC_PRECISION_BF16X6
+    bf16x8 A0, A1, A2;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      __bf16 h, m, l;
+      split3(a[j], h, m, l);
+      A0[j] = h;
+      A1[j] = m;
+      A2[j] = l;
+    }
+#pragma unroll
+    for (int s = 0; s < NSUB; ++s)
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+        bf16x8 B0, B1, B2;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          __bf16 h, m, l;
+          split3(comp<VEC>(breg[s][j], v), h, m, l);
+          B0[j] = h;
+          B1[j] = m;
+          B2[j] = l;
+        }
+        floatx4 c = acc[s][v];
+        if (has_vals) {
+          c = MFMA_BF16(A2, B0, c);
+          c = MFMA_BF16(A1, B1, c);
+        }
+        c = MFMA_BF16(A0, B2, c);
+        if (has_vals) c = MFMA_BF16(A1, B0, c);
+        c = MFMA_BF16(A0, B1, c);
+        c = MFMA_BF16(A0, B0, c);
+        acc[s][v] = c;
+      }
+  }
+}
+
+// Accumulates the TC blocks [b0, b1) of one window into acc.
+template <int PREC, int VEC, int NSUB>
+__device__ __forceinline__ void run_blocks(const SpmmParams &p, int b0, int b1, int colbase,
+                                           float *lds, floatx4 (&acc)[NSUB][VEC],
+                                           bool has_vals) {
+  typedef typename VecType<VEC>::type vfloat;
+  const int lane = lane_id();
+  const int g = lane >> 4;
+  const int q = lane & 15;
+  const int64_t N = p.N;
+  for (int bb = b0; bb < b1; bb += kGroup) {
+    const int nblk = min(kGroup, b1 - bb);
+    // Gather the 8 B rows of block g (rows of blocks past the run read 0).
+    int rows[8];
+    if (g < nblk) {
+      const int32_t *ap = p.atox + (int64_t)(bb + g) * kBlkW;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) rows[j] = ap[j];
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) rows[j] = -1;
+    }
+    vfloat breg[NSUB][8];
+#pragma unroll
+    for (int s = 0; s < NSUB; ++s) {
+      const int col = colbase + s * 16 * VEC;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int r = rows[j];
+        if ((unsigned)r < (unsigned)p.b_rows && col < p.N)
+          breg[s][j] = *reinterpret_cast<const vfloat *>(p.B + (int64_t)r * N + col);
+        else
+          breg[s][j] = vzero<VEC>();
+      }
+    }
+    // Dense 16x8 A tiles of the group in this wave's LDS.
+    int o[kGroup + 1];
+#pragma unroll
+    for (int k = 0; k <= kGroup; ++k) o[k] = p.off[bb + min(k, nblk)];
+    floatx4 *t4 = reinterpret_cast<floatx4 *>(lds);
+    const floatx4 z4 = {0.f, 0.f, 0.f, 0.f};
+    t4[lane * 2] = z4;
+    t4[lane * 2 + 1] = z4;
+    wave_lds_sync();
+    const int e_end = min(o[kGroup], p.nnz);
+    for (int e = o[0] + lane; e < e_end; e += 64) {
+      const int gg = (e >= o[1]) + (e >= o[2]) + (e >= o[3]);
+      const float v = has_vals ? p.vals[e] : 1.0f;
+      lds[gg * (kBlkH * kBlkW) + (p.tile[e] & (kBlkH * kBlkW - 1))] = v;
+    }
+    wave_lds_sync();
+    float a[8];
+    {
+      const floatx4 lo = t4[(g * (kBlkH * kBlkW) + q * kBlkW) / 4];
+      const floatx4 hi = t4[(g * (kBlkH * kBlkW) + q * kBlkW) / 4 + 1];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        a[j] = lo[j];
+        a[j + 4] = hi[j];
+      }
+    }
+    wave_lds_sync();
+    mfma_group<PREC, VEC, NSUB>(a, breg, acc, has_vals);
+  }
+}
+
+template <int VEC, int NSUB>
+__device__ __forceinline__ void zero_acc(floatx4 (&acc)[NSUB][VEC]) {
+#pragma unroll
+  for (int s = 0; s < NSUB; ++s)
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) acc[s][v] = floatx4{0.f, 0.f, 0.f, 0.f};
+}
+
+// Writes (or atomically adds) the 16 rows of window w held in acc.
+template <int VEC, int NSUB, bool ATOMIC>
+__device__ __forceinline__ void store_window(const SpmmParams &p, int w, int colbase,
+                                             const floatx4 (&acc)[NSUB][VEC]) {
+  typedef typename VecType<VEC>::type vfloat;
+  const int lane = lane_id();
+  const int g = lane >> 4;
+  const int64_t N = p.N;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int64_t row = (int64_t)w * kBlkH + 4 * g + i;
+    if (row < 0 || row >= p.M) continue;
+#pragma unroll
+    for (int s = 0; s < NSUB; ++s) {
+      const int col = colbase + s * 16 * VEC;
+      if (col >= p.N) continue;
+      float *dst = p.C + (int64_t)row * N + col;
+      if constexpr (ATOMIC) {
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) unsafeAtomicAdd(dst + v, acc[s][v][i]);
+      } else {
+        vfloat out;
+        if constexpr (VEC == 1) out = acc[s][0][i];
+        else
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) out[v] = acc[s][v][i];
+        __builtin_nontemporal_store(out, reinterpret_cast<vfloat *>(dst));
+      }
+    }
+  }
+}
+
+// First block index in [b, c1) whose window differs from w (c1 if none).
+__device__ __forceinline__ int segment_end(const int32_t *rowid, int b, int c1, int w) {
+  const int lane = lane_id();
+  for (int base = b; base < c1; base += 64) {
+    const int idx = base + lane;
+    const int rw = idx < c1 ? rowid[idx] : -1;
+    const uint64_t m = __ballot(rw != w);
+    if (m) return base + (int)__builtin_ctzll(m);
+  }
+  return c1;
+}
+
+template <int PREC, int VEC, int NSUB, bool BALANCED>
+__device__ __forceinline__ void run_unit(const SpmmParams &p, int item, int ct, float *lds) {
+  const int lane = lane_id();
+  const int colbase = ct * (16 * VEC * NSUB) + (lane & 15) * VEC;
+  const bool has_vals = p.vals != nullptr;
+  floatx4 acc[NSUB][VEC];
+  if constexpr (!BALANCED) {
+    const int w = item;
+    // Offsets are clamped to the blocks actually passed in.
+    const int b1 = max(0, min(p.rwo[w + 1], p.num_blocks));
+    const int b0 = max(0, min(p.rwo[w], b1));
+    zero_acc<VEC, NSUB>(acc);
+    run_blocks<PREC, VEC, NSUB>(p, b0, b1, colbase, lds, acc, has_vals);
+    store_window<VEC, NSUB, false>(p, w, colbase, acc);
+  } else {
+    const int c0 = item * p.chunk;
+    const int c1 = min(c0 + p.chunk, p.num_blocks);
+    int b = c0;
+    while (b < c1) {
+      const int w = wave_uniform(p.rowid[b]);
+      const int se = wave_uniform(segment_end(p.rowid, b, c1, w));
+      zero_acc<VEC, NSUB>(acc);
+      run_blocks<PREC, VEC, NSUB>(p, b, se, colbase, lds, acc, has_vals);
+      const bool start_full = (b == 0) || (p.rowid[b - 1] != w);
+      const bool end_full = (se == p.num_blocks) || (p.rowid[se] != w);
+      if (start_full && end_full) store_window<VEC, NSUB, false>(p, w, colbase, acc);
+      else store_window<VEC, NSUB, true>(p, w, colbase, acc);
+      b = se;
+    }
+  }
+}
+
+// Workgroup i serves column tile i % n_ct.  Workgroups are dispatched to the
+// 8 XCDs round robin, so when n_ct divides 8 every XCD (and its L2) only
+// sees one column tile of B: 1/n_ct of each gathered row.
+template <int PREC, int VEC, int NSUB, bool BALANCED>
+__global__ __launch_bounds__(256) void spmm_kernel(SpmmParams p) {
+  __shared__ __attribute__((aligned(16))) float s_tile[kWavesPerBlock][kTileFloats];
+  const int wave = wave_uniform(threadIdx.x >> 6);
+  const int ct = blockIdx.x % p.n_ct;
+  const int item = (blockIdx.x / p.n_ct) * kWavesPerBlock + wave;
+  if (item >= p.num_items) return;
+  run_unit<PREC, VEC, NSUB, BALANCED>(p, item, ct, s_tile[wave]);
+}
+
+// Persistent variant of the balanced schedule: every wave pulls units from a
+// global counter until the list is exhausted.
+template <int PREC, int VEC, int NSUB>
+__global__ __launch_bounds__(256) void spmm_kernel_persistent(SpmmParams p) {
+  __shared__ __attribute__((aligned(16))) float s_tile[kWavesPerBlock][kTileFloats];
+  const int wave = wave_uniform(threadIdx.x >> 6);
+  const int lane = lane_id();
+  const int total = p.num_items * p.n_ct;
+  while (true) {
+    int u = 0;
+    if (lane == 0) u = atomicAdd(p.queue, 1);
+    u = __shfl(u, 0, 64);
+    u = wave_uniform(u);
+    if (u >= total) break;
+    run_unit<PREC, VEC, NSUB, true>(p, u / p.n_ct, u % p.n_ct, s_tile[wave]);
+  }
+}
+
+// Balanced-schedule preparation, one wave per window: zero the rows of
+// windows that no chunk stores whole (empty windows, windows spanning
+// several chunks) and reset the work counter.
+__global__ __launch_bounds__(256) void k_balance_prep(const int32_t *rowid, int32_t T,
+                                                      int32_t W, int32_t M, int32_t N,
+                                                      int32_t chunk, float *C, int32_t *queue) {
+  if (queue && blockIdx.x == 0 && threadIdx.x == 0) *queue = 0;
+  const int lane = threadIdx.x & 63;
+  const int64_t w = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (w >= W) return;
+  const int fb = lower_bound_dev<int32_t>(rowid, 0, T, (int32_t)w);
+  const int lb = lower_bound_dev<int32_t>(rowid, fb, T, (int32_t)(w + 1));
+  const bool zero = (fb == lb) || (fb / chunk != (lb - 1) / chunk);
+  if (!zero) return;
+  const int64_t r0 = w * kBlkH;
+  const int64_t r1 = (r0 + kBlkH < (int64_t)M) ? r0 + kBlkH : (int64_t)M;
+  float *base = C + r0 * (int64_t)N;
+  const int64_t count = (r1 - r0) * (int64_t)N;
+  for (int64_t idx = lane; idx < count; idx += 64) base[idx] = 0.f;
+}
+
+// ------------------------------------------------------------ host side
+
+typedef void (*KernelFn)(SpmmParams);
+
+struct Variant {
+  int vec;
+  int nsub;
+};
+
+template <int PREC, int VEC, int NSUB>
+KernelFn pick_schedule(int schedule) {
+  switch (schedule) {
+    case 0: return spmm_kernel<PREC, VEC, NSUB, false>;
+    case 1: return spmm_kernel<PREC, VEC, NSUB, true>;
+    default: return spmm_kernel_persistent<PREC, VEC, NSUB>;
+  }
+}
+
+template <int PREC>
+KernelFn pick_shape(int vec, int nsub, int schedule) {
+  if (vec == 4 && nsub == 2) return pick_schedule<PREC, 4, 2>(schedule);
+  if (vec == 4) return pick_schedule<PREC, 4, 1>(schedule);
+  if (vec == 2) return pick_schedule<PREC, 2, 1>(schedule);
+  return pick_schedule<PREC, 1, 1>(schedule);
+}
+
+KernelFn pick_kernel(int prec, int vec, int nsub, int schedule) {
+  switch (prec) {
+    case DTC_PRECISION_FP32: return pick_shape<DTC_PRECISION_FP32>(vec, nsub, schedule);
+    case DTC_PRECISION_BF16: return pick_shape<DTC_PRECISION_BF16>(vec, nsub, schedule);
+    case DTC_PRECISION_FP16: return pick_shape<DTC_PRECISION_FP16>(vec, nsub, schedule);
+    case DTC_PRECISION_BF16X3: return pick_shape<DTC_PRECISION_BF16X3>(vec, nsub, schedule);
+    default: return pick_shape<DTC_PRECISION_BF16X6>(vec, nsub, schedule);
+  }
+}
+
+dtc_status_t resolve_shape(const dtc_spmm_options_t *opt, int32_t n, const float *b,
+                           const float *c, Variant *out) {
+  int vec = opt ? opt->vec : 0;
+  int nsub = opt ? opt->subtiles : 0;
+  if (vec != 0 && vec != 1 && vec != 2 && vec != 4)
+    return fail(DTC_ERROR_INVALID_VALUE, "vec must be 0, 1, 2 or 4");
+  if (nsub < 0 || nsub > 2) return fail(DTC_ERROR_INVALID_VALUE, "subtiles must be 0, 1 or 2");
+  auto fits = [&](int v) {
+    return n % v == 0 && ((uintptr_t)b % (4 * v)) == 0 && ((uintptr_t)c % (4 * v)) == 0;
+  };
+  if (vec == 0) vec = fits(4) ? 4 : (fits(2) ? 2 : 1);
+  if (!fits(vec))
+    return fail(DTC_ERROR_INVALID_VALUE,
+                "vec " + std::to_string(vec) + " needs N and the B/C base addresses to be multiples of it");
+  if (nsub == 0) nsub = (vec == 4 && n >= 128) ? 2 : 1;
+  if (nsub == 2 && vec != 4) nsub = 1;
+  out->vec = vec;
+  out->nsub = nsub;
+  return DTC_SUCCESS;
+}
+
+dtc_status_t check_common(const uint8_t *tile, const int32_t *off, const int32_t *atox,
+                          int32_t num_nodes, const float *b, int32_t b_rows, int32_t n, float *c,
+                          const dtc_spmm_options_t *opt) {
+  if (num_nodes < 0 || b_rows < 0) return fail(DTC_ERROR_INVALID_VALUE, "sizes must be >= 0");
+  if (n <= 0) return fail(DTC_ERROR_INVALID_VALUE, "N (feature width) must be >= 1");
+  if (!off) return fail(DTC_ERROR_INVALID_VALUE, "TCblock_offset is NULL");
+  if (num_nodes > 0 && !c) return fail(DTC_ERROR_INVALID_VALUE, "C is NULL");
+  if (b_rows > 0 && !b) return fail(DTC_ERROR_INVALID_VALUE, "B is NULL");
+  (void)tile;
+  (void)atox;
+  const int prec = opt ? opt->precision : DTC_PRECISION_FP32;
+  if (prec < DTC_PRECISION_FP32 || prec > DTC_PRECISION_BF16X6)
+    return fail(DTC_ERROR_INVALID_VALUE, "unknown precision");
+  if ((int64_t)num_nodes * n > ((int64_t)1 << 40))
+    return fail(DTC_ERROR_UNSUPPORTED, "output too large");
+  return DTC_SUCCESS;
+}
+
+}  // namespace
+}  // namespace dtc
+
+using namespace dtc;
+
+extern "C" {
+
+dtc_status_t dtc_spmm_windows(const int32_t *row_window_offset, const uint8_t *tcblocktile_id,
+                              const int32_t *tcblock_offset, const int32_t *sparse_atox,
+                              const float *tcblock_values, int32_t num_windows,
+                              int32_t num_blocks, int32_t nnz,
+                              int32_t num_nodes, const float *b, int32_t b_rows, int32_t n,
+                              float *c, const dtc_spmm_options_t *options,
+                              dtc_stream_t stream_) {
+  clear_error();
+  hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
+  dtc_status_t st = check_common(tcblocktile_id, tcblock_offset, sparse_atox, num_nodes, b,
+                                 b_rows, n, c, options);
+  if (st != DTC_SUCCESS) return st;
+  if (!row_window_offset) return fail(DTC_ERROR_INVALID_VALUE, "Rowwindow_offset is NULL");
+  if (num_windows < 0 || num_blocks < 0 || nnz < 0)
+    return fail(DTC_ERROR_INVALID_VALUE, "num_windows, num_blocks and nnz must be >= 0");
+  if (num_blocks > 0 && (!tcblocktile_id || !sparse_atox))
+    return fail(DTC_ERROR_INVALID_VALUE, "TCblocktile_id and sparse_AToX_idx are required");
+  if ((int64_t)num_windows * kBlkH < num_nodes)
+    return fail(DTC_ERROR_INVALID_VALUE,
+                "Rowwindow_offset has " + std::to_string(num_windows) + " windows, fewer than num_nodes/16");
+  Variant var;
+  st = resolve_shape(options, n, b, c, &var);
+  if (st != DTC_SUCCESS) return st;
+  // Only windows that hold output rows are processed.
+  const int32_t W = (int32_t)std::min<int64_t>(num_windows, ceil_div(num_nodes, kBlkH));
+  if (W == 0) return DTC_SUCCESS;
+  SpmmParams p{};
+  p.rwo = row_window_offset;
+  p.tile = tcblocktile_id;
+  p.off = tcblock_offset;
+  p.atox = sparse_atox;
+  p.vals = tcblock_values;
+  p.B = b;
+  p.C = c;
+  p.num_windows = W;
+  p.M = num_nodes;
+  p.b_rows = b_rows;
+  p.N = n;
+  p.n_ct = (int32_t)ceil_div(n, 16 * var.vec * var.nsub);
+  p.num_items = W;
+  p.num_blocks = num_blocks;
+  p.nnz = nnz;
+  const int prec = options ? options->precision : DTC_PRECISION_FP32;
+  KernelFn fn = pick_kernel(prec, var.vec, var.nsub, 0);
+  const int64_t grid = (int64_t)p.n_ct * ceil_div(W, kWavesPerBlock);
+  if (grid > INT32_MAX) return fail(DTC_ERROR_UNSUPPORTED, "grid too large");
+  hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(256), 0, stream, p);
+  DTC_LAUNCH_CHECK("spmm_windows");
+  return DTC_SUCCESS;
+}
+
+dtc_status_t dtc_spmm_balanced_workspace_size(int32_t num_blocks, size_t *bytes) {
+  clear_error();
+  if (!bytes) return fail(DTC_ERROR_INVALID_VALUE, "bytes is NULL");
+  (void)num_blocks;
+  *bytes = 256;
+  return DTC_SUCCESS;
+}
+
+dtc_status_t dtc_spmm_balanced(const int32_t *tcblock_rowid, const uint8_t *tcblocktile_id,
+                               const int32_t *tcblock_offset, const int32_t *sparse_atox,
+                               const float *tcblock_values, int32_t num_blocks, int32_t nnz,
+                               int32_t num_nodes, const float *b, int32_t b_rows, int32_t n,
+                               float *c, const dtc_spmm_options_t *options, void *workspace,
+                               size_t workspace_bytes, dtc_stream_t stream_) {
+  clear_error();
+  hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
+  dtc_status_t st = check_common(tcblocktile_id, tcblock_offset, sparse_atox, num_nodes, b,
+                                 b_rows, n, c, options);
+  if (st != DTC_SUCCESS) return st;
+  if (num_blocks < 0 || nnz < 0) return fail(DTC_ERROR_INVALID_VALUE, "num_blocks and nnz must be >= 0");
+  if (num_blocks > 0 && (!tcblock_rowid || !tcblocktile_id || !sparse_atox))
+    return fail(DTC_ERROR_INVALID_VALUE, "TCblock_rowid, TCblocktile_id and sparse_AToX_idx are required");
+  Variant var;
+  st = resolve_shape(options, n, b, c, &var);
+  if (st != DTC_SUCCESS) return st;
+  const int chunk = (options && options->chunk_blocks > 0) ? options->chunk_blocks : 64;
+  const bool persistent = options && options->persistent;
+  if (persistent && (!workspace || workspace_bytes < 256))
+    return fail(DTC_ERROR_WORKSPACE, "persistent balanced SpMM needs a 256-byte workspace");
+  const int32_t W = (int32_t)ceil_div(num_nodes, kBlkH);
+  if (W == 0) return DTC_SUCCESS;
+  SpmmParams p{};
+  p.rowid = tcblock_rowid;
+  p.tile = tcblocktile_id;
+  p.off = tcblock_offset;
+  p.atox = sparse_atox;
+  p.vals = tcblock_values;
+  p.B = b;
+  p.C = c;
+  p.num_windows = W;
+  p.num_blocks = num_blocks;
+  p.M = num_nodes;
+  p.b_rows = b_rows;
+  p.N = n;
+  p.nnz = nnz;
+  p.n_ct = (int32_t)ceil_div(n, 16 * var.vec * var.nsub);
+  p.num_items = (int32_t)ceil_div(num_blocks, chunk);
+  p.chunk = chunk;
+  p.queue = persistent ? static_cast<int32_t *>(workspace) : nullptr;
+  k_balance_prep<<<(unsigned)ceil_div(W, kWavesPerBlock), 256, 0, stream>>>(
+      tcblock_rowid, num_blocks, W, num_nodes, n, chunk, c, p.queue);
+  DTC_LAUNCH_CHECK("balance_prep");
+  if (num_blocks == 0) return DTC_SUCCESS;
+  const int prec = options ? options->precision : DTC_PRECISION_FP32;
+  if (!persistent) {
+    KernelFn fn = pick_kernel(prec, var.vec, var.nsub, 1);
+    const int64_t grid = (int64_t)p.n_ct * ceil_div(p.num_items, kWavesPerBlock);
+    if (grid > INT32_MAX) return fail(DTC_ERROR_UNSUPPORTED, "grid too large");
+    hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(256), 0, stream, p);
+  } else {
+    int dev = 0, cus = 0;
+    DTC_HIP_CHECK(hipGetDevice(&dev));
+    DTC_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int64_t units = (int64_t)p.num_items * p.n_ct;
+    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(ceil_div(units, kWavesPerBlock),
+                                                                (int64_t)cus * 8));
+    KernelFn fn = pick_kernel(prec, var.vec, var.nsub, 2);
+    hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(256), 0, stream, p);
+  }
+  DTC_LAUNCH_CHECK("spmm_balanced");
+  return DTC_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,193 @@
+/*
+ * dtc.h -- C ABI of libdtc, the MI355X (gfx950) DTC-SpMM library.
+ *
+ * Plain pointers and sizes only.  Every device buffer is owned by the caller;
+ * the library allocates no device memory.  Scratch memory is passed in as a
+ * workspace whose size comes from the matching *_workspace_size query.  All
+ * work is enqueued on the given stream (a hipStream_t; NULL is the default
+ * stream).  Functions return a dtc_status_t and never exit the process;
+ * dtc_last_error() gives a human-readable detail of the calling thread's
+ * last failure.
+ *
+ * Index types follow the reference: int32 CSR, uint8 tile ids, int32 block
+ * offsets and SparseAToX (DTC-SpMM/DTCSpMM.cpp:313-359).
+ */
+#ifndef DTC_H_
+#define DTC_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same type as hipStream_t. */
+typedef struct ihipStream_t *dtc_stream_t;
+
+typedef enum dtc_status {
+  DTC_SUCCESS = 0,
+  DTC_ERROR_INVALID_VALUE = 1, /* bad argument: null pointer, size, shape, plan name */
+  DTC_ERROR_UNSUPPORTED = 2,   /* well-formed but not supported (block shape, precision) */
+  DTC_ERROR_WORKSPACE = 3,     /* workspace missing or too small */
+  DTC_ERROR_INVALID_CSR = 4,   /* row pointer / column index check failed on the device */
+  DTC_ERROR_HIP = 5,           /* HIP runtime error */
+  DTC_ERROR_INCONSISTENT = 6   /* blockPartition was not zero for an empty row window */
+} dtc_status_t;
+
+typedef enum dtc_precision {
+  DTC_PRECISION_FP32 = 0,   /* float32 operands, v_mfma_f32_16x16x4_f32 */
+  DTC_PRECISION_BF16 = 1,   /* one bf16 pass, v_mfma_f32_16x16x32_bf16 */
+  DTC_PRECISION_FP16 = 2,   /* one fp16 pass, v_mfma_f32_16x16x32_f16 */
+  DTC_PRECISION_BF16X3 = 3, /* 2-limb bf16 split: hi*hi + hi*lo + lo*hi */
+  DTC_PRECISION_BF16X6 = 4  /* 3-limb bf16 split: all limb products of order <= 2 */
+} dtc_precision_t;
+
+typedef struct dtc_spmm_options {
+  int32_t precision;    /* dtc_precision_t */
+  int32_t vec;          /* B columns per lane per gathered row: 1, 2, 4 (0 = widest N allows) */
+  int32_t subtiles;     /* 1 or 2 column tiles of 16*vec per wave (0 = auto) */
+  int32_t chunk_blocks; /* balanced schedule: TC blocks per work chunk (0 = 64) */
+  int32_t persistent;   /* balanced schedule: 1 = persistent waves pulling chunks from a queue */
+} dtc_spmm_options_t;
+
+/* ------------------------------------------------------------------ misc */
+
+const char *dtc_status_string(dtc_status_t status);
+/* Detail of the calling thread's last non-success status ("" if none). */
+const char *dtc_last_error(void);
+/* Library version, major * 10000 + minor * 100 + patch. */
+int32_t dtc_version(void);
+
+/* --------------------------------------------------- ME-TCF construction
+ *
+ * Replaces DTCSpMM.preprocess_gpu (DTC-SpMM/DTCSpMM.cpp:313-359) and
+ * seg_sort_dequ (DTC-SpMM/DTCSpMM_kernel.cu:349-407).  The reference sizes its
+ * block-dependent outputs from a device-to-host read of the block count, so
+ * the build is split in the same place: phase 1 computes the window arrays
+ * and the block count, the caller allocates the block-sized outputs, phase 2
+ * fills them.  The workspace carries the sorted edge keys from phase 1 to
+ * phase 2 and must not be touched in between.
+ *
+ * Outputs are bit-identical to the reference's for the same inputs,
+ * including empty windows (blockPartition left as passed), a partial last
+ * window (num_nodes % blk_h != 0), zero padding of SparseAToX, and tile ids
+ * computed as (row % blk_h) * blk_w + column slot, stored as uint8.
+ */
+
+/* Workspace bytes for both phases. */
+dtc_status_t dtc_metcf_workspace_size(int32_t num_nodes, int32_t nnz, int32_t blk_h,
+                                      int32_t blk_w, size_t *bytes);
+
+/*
+ * Phase 1.  edge_list: CSR column indices [nnz]; node_pointer: CSR row
+ * pointer [num_nodes + 1].  In/out (caller-allocated, as the reference's
+ * callers do, scripts/DTCSpMM/run_DTC_SpMM.py:49-58): block_partition
+ * [ceil(num_nodes / blk_h)], zero-filled by the caller; edge_to_column [nnz];
+ * edge_to_row [nnz].  Out: row_window_offset [ceil(num_nodes / blk_h) + 1];
+ * *block_count (host) = number of TC blocks T.  Synchronises the stream
+ * twice (CSR check, then the block count), as the reference does
+ * (DTCSpMM_kernel.cu:378-380, :384-387).
+ */
+dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *node_pointer,
+                                     int32_t num_nodes, int32_t nnz, int32_t blk_h,
+                                     int32_t blk_w, int32_t *block_partition,
+                                     int32_t *edge_to_column, int32_t *edge_to_row,
+                                     int32_t *row_window_offset, int32_t *block_count,
+                                     void *workspace, size_t workspace_bytes,
+                                     dtc_stream_t stream);
+
+/*
+ * Phase 2 (same workspace, after phase 1).  Out: tcblock_rowid [T],
+ * tcblocktile_id [nnz] (uint8), tcblock_offset [T + 1], sparse_atox
+ * [T * blk_w].  Optional (NULL to skip, build extensions): values [nnz]
+ * (CSR order) -> tcblock_values [nnz] (TC-block order); tcblock_edge [nnz]
+ * receives the CSR edge index stored at each TC-block position.
+ */
+dtc_status_t dtc_metcf_build_blocks(const int32_t *node_pointer, const int32_t *edge_to_row,
+                                    const int32_t *row_window_offset, int32_t num_nodes,
+                                    int32_t nnz, int32_t blk_h, int32_t blk_w,
+                                    int32_t block_count, const float *values,
+                                    int32_t *tcblock_rowid, uint8_t *tcblocktile_id,
+                                    int32_t *tcblock_offset, int32_t *sparse_atox,
+                                    float *tcblock_values, int32_t *tcblock_edge,
+                                    void *workspace, size_t workspace_bytes,
+                                    dtc_stream_t stream);
+
+/* ------------------------------------------------------------ SpMM forward
+ *
+ * C[M x N] = A * B with A in ME-TCF at 16 x 8 blocks, B [b_rows x N] float32
+ * row-major, C [num_nodes x N] float32 row-major.  tcblock_values (TC-block
+ * order, [nnz]) may be NULL: every stored element of A is then 1, as the
+ * reference's launchers pass (DTCSpMM_kernel.cu:837, :1140).  Columns of A
+ * that are >= b_rows read as zero (the dense_bound guard of
+ * DTCSpMM_kernel.cu:2735).  Any N >= 1 and any num_nodes are handled; rows of
+ * a partial last window are not written past num_nodes.  num_blocks (T) and
+ * nnz are the lengths of the block arrays and of tcblocktile_id; block ranges
+ * and element offsets read from the format are clamped to them.
+ */
+
+/*
+ * Unbalanced schedule: one work unit per (row window, column tile).
+ * Replaces run_DTCSpMM / spmm_forward_improved_ptx_uint8_cuda
+ * (DTCSpMM.cpp:117-131, DTCSpMM_kernel.cu:811-944) and DTCSpMM_gcn
+ * (DTCSpMM.cpp:147-161, DTCSpMM_kernel.cu:1033-1116).  No workspace; every
+ * output element is written (rows of empty windows get zeros).
+ */
+dtc_status_t dtc_spmm_windows(const int32_t *row_window_offset, const uint8_t *tcblocktile_id,
+                              const int32_t *tcblock_offset, const int32_t *sparse_atox,
+                              const float *tcblock_values, int32_t num_windows,
+                              int32_t num_blocks, int32_t nnz,
+                              int32_t num_nodes, const float *b, int32_t b_rows, int32_t n,
+                              float *c, const dtc_spmm_options_t *options,
+                              dtc_stream_t stream);
+
+/*
+ * Balanced schedule: work chunks of options->chunk_blocks consecutive TC
+ * blocks, each split at row-window boundaries; windows that span several
+ * chunks are merged with float32 atomic adds into rows this call zeroes
+ * first.  Replaces run_DTCSpMM_balance /
+ * spmm_balance_forward_cuda_ptx_unit8_prefetch (DTCSpMM.cpp:163-178,
+ * DTCSpMM_kernel.cu:1118-1199).  Every output element is written.
+ */
+dtc_status_t dtc_spmm_balanced_workspace_size(int32_t num_blocks, size_t *bytes);
+dtc_status_t dtc_spmm_balanced(const int32_t *tcblock_rowid, const uint8_t *tcblocktile_id,
+                               const int32_t *tcblock_offset, const int32_t *sparse_atox,
+                               const float *tcblock_values, int32_t num_blocks, int32_t nnz,
+                               int32_t num_nodes, const float *b, int32_t b_rows, int32_t n,
+                               float *c, const dtc_spmm_options_t *options, void *workspace,
+                               size_t workspace_bytes, dtc_stream_t stream);
+
+/* ---------------------------------------------------------------- backward
+ *
+ * The reference binds its "backward" names to the forward functions
+ * (DTCSpMM.cpp:384-399), i.e. it returns A * dY.  These entry points give the
+ * true gradients of C = A * B: dB = A^T * dC through the ME-TCF of A^T, and
+ * dA = SDDMM(dC, B) on A's pattern.
+ */
+
+/* Workspace bytes for dtc_csr_transpose. */
+dtc_status_t dtc_csr_transpose_workspace_size(int32_t num_rows, int32_t num_cols, int32_t nnz,
+                                              size_t *bytes);
+/* CSR [num_rows x num_cols] -> CSR of the transpose [num_cols x num_rows],
+ * columns ascending per row; perm[p] = source edge of transposed edge p. */
+dtc_status_t dtc_csr_transpose(const int32_t *node_pointer, const int32_t *edge_list,
+                               int32_t num_rows, int32_t num_cols, int32_t nnz,
+                               int32_t *node_pointer_t, int32_t *edge_list_t, int32_t *perm,
+                               void *workspace, size_t workspace_bytes, dtc_stream_t stream);
+
+/* out[e] = sum_j x[row(e), j] * y[col(e), j] for every CSR edge e
+ * (x: [num_rows x n], y: [y_rows x n]; columns >= y_rows give 0). */
+dtc_status_t dtc_sddmm_csr(const int32_t *node_pointer, const int32_t *edge_list,
+                           int32_t num_rows, int32_t nnz, const float *x, const float *y,
+                           int32_t y_rows, int32_t n, float *out, dtc_stream_t stream);
+
+/* out[p] = in[idx[p]] for p < count (moves values between CSR and TC-block order). */
+dtc_status_t dtc_gather_f32(const float *in, const int32_t *idx, int32_t count, float *out,
+                            dtc_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DTC_H_ */

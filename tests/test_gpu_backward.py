@@ -1,0 +1,119 @@
+"""Backward pass: GPU CSR transpose and SDDMM vs the oracle, and the
+autograd gradients of C = A(values) * B against float64 references (the
+oracle, a torch.sparse float64 product, and finite differences of the
+float64 function on a 2,048-node R-MAT instance)."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from helpers import random_csr, say
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ext():
+    from dtc import load_extension
+    return load_extension()
+
+
+def test_transpose_vs_oracle(ext):
+    rng = np.random.default_rng(31)
+    for case, (M, K) in enumerate([(1, 1), (50, 70), (333, 100), (1000, 4000)]):
+        rowptr, col = random_csr(rng, M, K, 6.0, sorted_rows=bool(case % 2), empty_frac=0.2)
+        say("transpose", M, K)
+        rpt, ct, perm = ext.csr_transpose(torch.from_numpy(rowptr).cuda(), torch.from_numpy(col).cuda(), M, K)
+        r2, c2, p2 = oracle.csr_transpose(rowptr, col, M, K)
+        assert np.array_equal(rpt.cpu().numpy(), r2)
+        assert np.array_equal(ct.cpu().numpy(), c2)
+        assert np.array_equal(perm.cpu().numpy(), p2)
+
+
+@pytest.mark.parametrize("N", [1, 3, 16, 64, 130])
+def test_sddmm_vs_oracle(ext, N):
+    rng = np.random.default_rng(N)
+    M, K = 700, 500
+    rowptr, col = random_csr(rng, M, K, 8.0, empty_frac=0.1)
+    X = rng.standard_normal((M, N)).astype(np.float32)
+    Y = rng.standard_normal((K, N)).astype(np.float32)
+    say("sddmm", N)
+    out = ext.sddmm_csr(torch.from_numpy(rowptr).cuda(), torch.from_numpy(col).cuda(),
+                        torch.from_numpy(X).cuda(), torch.from_numpy(Y).cuda()).cpu().numpy()
+    ref = oracle.sddmm_csr_f64(rowptr, col, X, Y, M)
+    rows = np.repeat(np.arange(M), np.diff(rowptr))
+    scale = (np.abs(X[rows]) * np.abs(Y[col])).sum(1)
+    assert np.all(np.abs(out - ref) <= 1e-6 * scale + 1e-30)
+
+
+def _grads_f64(rowptr, col, vals, B, dC, M, K):
+    """dvalues and dB of <C, dC> with C = A B, via a torch.sparse float64 product."""
+    rows = np.repeat(np.arange(M), np.diff(rowptr))
+    idx = torch.from_numpy(np.stack([rows, col]).astype(np.int64))
+    v = torch.from_numpy(vals.astype(np.float64)).requires_grad_(True)
+    Bt = torch.from_numpy(B.astype(np.float64)).requires_grad_(True)
+    A = torch.sparse_coo_tensor(idx, v, (M, K))
+    C = torch.sparse.mm(A, Bt)
+    (C * torch.from_numpy(dC.astype(np.float64))).sum().backward()
+    return v.grad.numpy(), Bt.grad.numpy(), C.detach().numpy()
+
+
+@pytest.mark.parametrize("schedule", ["windows", "balanced"])
+def test_autograd_square_and_rect(ext, schedule):
+    from dtc.matrix import DTCMatrix, spmm_autograd
+    rng = np.random.default_rng(41)
+    for M, K, N in [(300, 300, 32), (257, 400, 20), (400, 130, 64)]:
+        rowptr, col = random_csr(rng, M, K, 7.0, sorted_rows=False, empty_frac=0.15)
+        vals = rng.uniform(-1, 1, col.shape[0]).astype(np.float32)
+        B = rng.standard_normal((K, N)).astype(np.float32)
+        dC = rng.standard_normal((M, N)).astype(np.float32)
+        say("autograd", schedule, M, K, N)
+        A = DTCMatrix(rowptr, col, None, num_rows=M, num_cols=K)
+        v = torch.from_numpy(vals).cuda().requires_grad_(True)
+        Bt = torch.from_numpy(B).cuda().requires_grad_(True)
+        C = spmm_autograd(v, Bt, A, schedule=schedule)
+        C.backward(torch.from_numpy(dC).cuda())
+        gv, gB, C64 = _grads_f64(rowptr, col, vals, B, dC, M, K)
+        assert np.allclose(C.detach().cpu().numpy(), C64, rtol=1e-4, atol=1e-5)
+        assert np.allclose(v.grad.cpu().numpy(), gv, rtol=1e-4, atol=1e-4)
+        assert np.allclose(Bt.grad.cpu().numpy(), gB, rtol=1e-4, atol=1e-5)
+
+
+def test_gradients_rmat_2048_finite_differences(ext):
+    """GCN-style instance: R-MAT scale 11 (2,048 nodes), symmetric
+    normalisation.  Analytic float32 gradients vs central finite differences
+    of the float64 function along random directions."""
+    from dtc.graphs import rmat
+    from dtc.matrix import DTCMatrix, spmm_autograd
+    a = rmat(11, 16, seed=1)
+    M = a.num_rows
+    N = 64
+    rng = np.random.default_rng(5)
+    B = rng.standard_normal((M, N)).astype(np.float32)
+    dC = rng.standard_normal((M, N)).astype(np.float32)
+    say("rmat2048 gradients")
+    A = DTCMatrix(a.rowptr, a.col, None, num_rows=M, num_cols=M)
+    v = torch.from_numpy(a.values).cuda().requires_grad_(True)
+    Bt = torch.from_numpy(B).cuda().requires_grad_(True)
+    C = spmm_autograd(v, Bt, A)
+    C.backward(torch.from_numpy(dC).cuda())
+    gv = v.grad.double().cpu().numpy()
+    gB = Bt.grad.double().cpu().numpy()
+
+    def f(vals, Bm):
+        C64 = oracle.spmm_csr_f64(a.rowptr, a.col, vals.astype(np.float32), Bm.astype(np.float32), M)
+        return float((C64 * dC).sum())
+
+    # f is bilinear, so central differences are exact up to float64 rounding.
+    for trial in range(3):
+        dv = rng.standard_normal(a.nnz)
+        dB = rng.standard_normal(B.shape)
+        h = 1e-3
+        vals64 = a.values.astype(np.float64)
+        B64 = B.astype(np.float64)
+        fd_v = (f(vals64 + h * dv, B64) - f(vals64 - h * dv, B64)) / (2 * h)
+        fd_B = (f(vals64, B64 + h * dB) - f(vals64, B64 - h * dB)) / (2 * h)
+        an_v = float(gv @ dv)
+        an_B = float((gB * dB).sum())
+        assert abs(fd_v - an_v) <= 1e-3 * max(1.0, abs(fd_v)), (trial, fd_v, an_v)
+        assert abs(fd_B - an_B) <= 1e-3 * max(1.0, abs(fd_B)), (trial, fd_B, an_B)

@@ -1,0 +1,136 @@
+"""GPU ME-TCF build vs the CPU oracle: bit-exact on every output array."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from helpers import assert_metcf_equal, gpu_metcf, load_fixture_18, naive_metcf, random_csr, say
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ext():
+    from dtc import load_extension
+    return load_extension()
+
+
+def check(ext, rowptr, col, M, bh=16, bw=8, values=None, ctx=""):
+    say("metcf", ctx, f"M={M} nnz={col.shape[0]} bh={bh} bw={bw}")
+    g = gpu_metcf(ext, rowptr, col, M, bh, bw, values=values)
+    o = oracle.metcf(rowptr, col, M, bh, bw, values=values)
+    assert_metcf_equal(g, o, ctx=ctx)
+    if values is not None:
+        assert np.array_equal(g["TCblockValues"], o["TCblockValues"]), ctx
+    # edge order: the CSR edge stored at each TC position
+    assert np.array_equal(np.sort(g["TCblockEdge"]), np.arange(col.shape[0])), ctx
+    return g
+
+
+def test_fixture_18(ext):
+    rowptr, col, M, fx = load_fixture_18()
+    g = check(ext, rowptr, col, M, ctx="fixture18")
+    for k, v in fx["expected"].items():
+        if k in g:
+            assert list(g[k]) == v, k
+
+
+@pytest.mark.parametrize("bh,bw", [(16, 8), (16, 16), (8, 8), (16, 4), (16, 32), (1, 8), (32, 8)])
+def test_random_small(ext, bh, bw):
+    rng = np.random.default_rng(50 + bh * 7 + bw)
+    for case in range(12):
+        M = int(rng.integers(1, 90))
+        rowptr, col = random_csr(rng, M, int(rng.integers(1, 120)), float(rng.uniform(0, 14)),
+                                 sorted_rows=bool(case % 2), empty_frac=0.3 if case % 3 == 0 else 0.0,
+                                 dup_frac=0.2 if case % 4 == 0 else 0.0)
+        vals = rng.standard_normal(col.shape[0]).astype(np.float32)
+        check(ext, rowptr, col, M, bh, bw, values=vals, ctx=f"random bh={bh} bw={bw} case={case}")
+
+
+def test_c1_uniform(ext):
+    from dtc.graphs import uniform_rows
+    a = uniform_rows(4096, 4096, 16, seed=0)
+    g = check(ext, a.rowptr, a.col, a.num_rows, values=a.values, ctx="C1")
+    assert g["block_count"] == 8081
+
+
+@pytest.mark.parametrize("scale", [10, 14])
+def test_rmat(ext, scale):
+    from dtc.graphs import rmat
+    a = rmat(scale, 16, seed=1)
+    check(ext, a.rowptr, a.col, a.num_rows, values=a.values, ctx=f"rmat{scale}")
+
+
+def test_sbm_hubs_small(ext):
+    from dtc.graphs import sbm_hubs
+    a = sbm_hubs(20000, communities=20, avg_per_row=40, hubs=6, hub_min=3000, hub_max=8000, seed=2)
+    check(ext, a.rowptr, a.col, a.num_rows, values=a.values, ctx="sbm")
+
+
+def test_banded_and_block_diagonal(ext):
+    from dtc.graphs import banded, block_diagonal
+    a = banded(3000, 40, seed=1)
+    check(ext, a.rowptr, a.col, a.num_rows, ctx="banded")
+    b = block_diagonal(5000, 100, seed=2)
+    check(ext, b.rowptr, b.col, b.num_rows, ctx="blockdiag")
+
+
+def test_window_beyond_reference_smem_limit(ext):
+    # one window with more than 16,383 TC blocks overflows the reference's
+    # 128 KB shared-memory opt-in (DTCSpMM_kernel.cu:311-325); no limit here.
+    rng = np.random.default_rng(9)
+    M, K = 16, 1 << 20
+    rows = [np.sort(rng.choice(K, 12000, replace=False)) for _ in range(M)]
+    rowptr = np.concatenate([[0], np.cumsum([r.shape[0] for r in rows])]).astype(np.int32)
+    col = np.concatenate(rows).astype(np.int32)
+    g = check(ext, rowptr, col, M, ctx="huge-window")
+    assert g["block_count"] > 16383
+
+
+def test_unsorted_rows_and_duplicates(ext):
+    rng = np.random.default_rng(11)
+    rowptr, col = random_csr(rng, 1000, 500, 20.0, sorted_rows=False, dup_frac=0.5)
+    check(ext, rowptr, col, 1000, ctx="unsorted+dups")
+
+
+def test_empty_and_degenerate(ext):
+    for M in [0, 1, 15, 16, 17, 100]:
+        rowptr = np.zeros(M + 1, np.int32)
+        col = np.zeros(0, np.int32)
+        check(ext, rowptr, col, M, ctx=f"empty M={M}")
+    rowptr = np.array([0, 0, 0, 1], np.int32)
+    check(ext, rowptr, np.array([2], np.int32), 3, ctx="single edge")
+
+
+def test_inconsistent_block_partition_raises(ext):
+    rowptr, col, M, _ = load_fixture_18()
+    M2 = 40
+    rowptr2 = np.concatenate([rowptr, np.full(M2 - M, rowptr[-1], np.int32)])
+    bp = np.zeros(3, np.int32)
+    bp[2] = 5
+    say("metcf inconsistent blockPartition")
+    with pytest.raises(RuntimeError, match="blockPartition"):
+        gpu_metcf(ext, rowptr2, col, M2, block_partition=bp)
+
+
+def test_invalid_csr_raises(ext):
+    say("metcf invalid CSR")
+    with pytest.raises(RuntimeError, match="invalid CSR"):
+        gpu_metcf(ext, np.array([0, 3, 2, 4], np.int32), np.array([0, 1, 2, 3], np.int32), 3)
+    with pytest.raises(RuntimeError, match="invalid CSR"):
+        gpu_metcf(ext, np.array([0, 2, 4], np.int32), np.array([0, -1, 2, 3], np.int32), 2)
+    with pytest.raises(RuntimeError, match="invalid CSR"):
+        gpu_metcf(ext, np.array([0, 2, 3], np.int32), np.array([0, 1, 2, 3], np.int32), 2)
+
+
+def test_wrong_sizes_raise(ext):
+    dev = "cuda"
+    i32 = torch.int32
+    col = torch.zeros(4, dtype=i32, device=dev)
+    rpt = torch.tensor([0, 2, 4], dtype=i32, device=dev)
+    with pytest.raises(RuntimeError, match="blockPartition"):
+        ext.preprocess_gpu(col, rpt, 2, 16, 8, torch.zeros(3, dtype=i32, device=dev),
+                           torch.zeros(4, dtype=i32, device=dev), torch.zeros(4, dtype=i32, device=dev))
+    with pytest.raises(RuntimeError, match="dtype"):
+        ext.preprocess_gpu(col.long(), rpt, 2, 16, 8, torch.zeros(1, dtype=i32, device=dev),
+                           torch.zeros(4, dtype=i32, device=dev), torch.zeros(4, dtype=i32, device=dev))

@@ -1,0 +1,205 @@
+"""GPU SpMM vs the float64 CSR oracle, for both schedules, every precision,
+vector shapes and N, plus the reference's own call sequence.
+
+Tolerances (componentwise: max |C - C64| / (|A| |B|), float64 denominators):
+  fp32    4e-6   float32 operands and accumulation
+  bf16x6  4e-6   3-limb bf16 split, products of limb order <= 2
+  bf16x3  1e-4   2-limb split, hi*hi + hi*lo + lo*hi (dropped terms ~3 * 2^-16)
+  fp16    4e-3   single fp16 pass, unit roundoff 2^-11 per operand
+  bf16    3e-2   single bf16 pass, unit roundoff 2^-8 per operand
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from helpers import gpu_metcf, load_fixture_18, random_csr, say
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp32": 4e-6, "bf16x6": 4e-6, "bf16x3": 1e-4, "fp16": 4e-3, "bf16": 3e-2}
+PLANS = ["float_nonsplit", "float2_nonsplit", "float2_split", "float4_nonsplit", "float4_split"]
+
+
+@pytest.fixture(scope="module")
+def ext():
+    from dtc import load_extension
+    return load_extension()
+
+
+def componentwise(C, C64, Cabs):
+    C = C.astype(np.float64)
+    d = np.abs(C - C64)
+    m = Cabs > 0
+    e = float((d[m] / Cabs[m]).max()) if m.any() else 0.0
+    z = float(d[~m].max()) if (~m).any() else 0.0
+    return max(e, z)
+
+
+class Case:
+    def __init__(self, ext, rowptr, col, M, vals=None, name=""):
+        self.rowptr, self.col, self.M, self.vals, self.name = rowptr, col, M, vals, name
+        say("build", name, f"M={M} nnz={col.shape[0]}")
+        self.m = gpu_metcf(ext, rowptr, col, M, values=vals)
+        self.dev = self.m["_dev"]
+        self.tcv = self.dev[6] if vals is not None else None
+
+    def run(self, ext, B, schedule, precision, use_vals=True, **kw):
+        d = self.dev
+        idx = d[0] if schedule == "windows" else d[1]
+        v = self.tcv if use_vals else None
+        return ext.spmm(B, idx, d[2], d[3], d[4], self.M, schedule=schedule, values=v,
+                        precision=precision, **kw)
+
+    def ref(self, Bn, use_vals=True):
+        v = self.vals if use_vals else None
+        return oracle.spmm_csr_f64(self.rowptr, self.col, v, Bn, self.M, with_abs=True)
+
+
+@pytest.fixture(scope="module")
+def c1(ext):
+    from dtc.graphs import uniform_rows
+    a = uniform_rows(4096, 4096, 16, seed=0)
+    return Case(ext, a.rowptr, a.col, a.num_rows, a.values, "C1")
+
+
+@pytest.fixture(scope="module")
+def rmat12(ext):
+    from dtc.graphs import rmat
+    a = rmat(12, 16, seed=1)
+    return Case(ext, a.rowptr, a.col, a.num_rows, a.values, "rmat12")
+
+
+@pytest.fixture(scope="module")
+def ragged(ext):
+    rng = np.random.default_rng(21)
+    M = 1000 + 7  # partial last window
+    rowptr, col = random_csr(rng, M, M, 9.0, sorted_rows=False, empty_frac=0.25)
+    # a few very long rows so windows span many chunks
+    extra_rows = [3, 500, 1006]
+    rows = [list(col[rowptr[r]:rowptr[r + 1]]) for r in range(M)]
+    for r in extra_rows:
+        rows[r] = sorted(set(rows[r]) | set(rng.choice(M, 700, replace=False).tolist()))
+    rowptr = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int32)
+    col = np.concatenate([np.array(r, np.int32) for r in rows]).astype(np.int32)
+    vals = rng.uniform(-1, 1, col.shape[0]).astype(np.float32)
+    return Case(ext, rowptr, col, M, vals, "ragged")
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x6", "bf16x3", "fp16", "bf16"])
+@pytest.mark.parametrize("schedule", ["windows", "balanced"])
+@pytest.mark.parametrize("N", [16, 128, 256])
+def test_precisions_c1(ext, c1, precision, schedule, N):
+    rng = np.random.default_rng(N)
+    Bn = rng.uniform(-1, 1, (c1.M, N)).astype(np.float32)
+    B = torch.from_numpy(Bn).cuda()
+    for use_vals in (True, False):
+        say("spmm C1", precision, schedule, N, use_vals)
+        C = c1.run(ext, B, schedule, precision, use_vals=use_vals).cpu().numpy()
+        C64, Cabs = c1.ref(Bn, use_vals)
+        err = componentwise(C, C64, Cabs)
+        assert err <= TOL[precision], (precision, schedule, N, use_vals, err)
+
+
+@pytest.mark.parametrize("N", [1, 3, 5, 8, 17, 32, 64, 100, 128, 260, 512])
+@pytest.mark.parametrize("schedule", ["windows", "balanced"])
+def test_shapes_ragged(ext, ragged, N, schedule):
+    rng = np.random.default_rng(1000 + N)
+    Bn = rng.standard_normal((ragged.M, N)).astype(np.float32)
+    B = torch.from_numpy(Bn).cuda()
+    C64, Cabs = ragged.ref(Bn)
+    for vec, sub in [(0, 0), (1, 1), (2, 1), (4, 1), (4, 2)]:
+        say("spmm ragged", schedule, N, vec, sub)
+        C = ragged.run(ext, B, schedule, "fp32", vec=vec, subtiles=sub, chunk_blocks=5).cpu().numpy()
+        err = componentwise(C, C64, Cabs)
+        assert err <= TOL["fp32"], (schedule, N, vec, sub, err)
+
+
+@pytest.mark.parametrize("chunk", [1, 2, 3, 4, 5, 8, 64, 1000])
+@pytest.mark.parametrize("persistent", [False, True])
+def test_balanced_chunks(ext, ragged, rmat12, chunk, persistent):
+    for case in (ragged, rmat12):
+        rng = np.random.default_rng(chunk)
+        Bn = rng.standard_normal((case.M, 64)).astype(np.float32)
+        B = torch.from_numpy(Bn).cuda()
+        say("spmm chunks", case.name, chunk, persistent)
+        C = case.run(ext, B, "balanced", "fp32", chunk_blocks=chunk, persistent=persistent).cpu().numpy()
+        C64, Cabs = case.ref(Bn)
+        assert componentwise(C, C64, Cabs) <= TOL["fp32"], (case.name, chunk, persistent)
+
+
+def test_schedules_agree_rmat(ext, rmat12):
+    rng = np.random.default_rng(5)
+    Bn = rng.standard_normal((rmat12.M, 128)).astype(np.float32)
+    B = torch.from_numpy(Bn).cuda()
+    say("spmm rmat12 schedules")
+    Cw = rmat12.run(ext, B, "windows", "fp32")
+    Cb = rmat12.run(ext, B, "balanced", "fp32", chunk_blocks=16)
+    C64, Cabs = rmat12.ref(Bn)
+    assert componentwise(Cw.cpu().numpy(), C64, Cabs) <= TOL["fp32"]
+    assert componentwise(Cb.cpu().numpy(), C64, Cabs) <= TOL["fp32"]
+    assert torch.allclose(Cw, Cb, rtol=1e-5, atol=1e-6)
+
+
+def test_repeated_launches_identical(ext, ragged):
+    # The reference's balanced op accumulates into its output on every one of
+    # its 1000 timed launches; every launch here starts from the zeroed rows.
+    rng = np.random.default_rng(8)
+    B = torch.from_numpy(rng.standard_normal((ragged.M, 32)).astype(np.float32)).cuda()
+    say("spmm repeat")
+    out = torch.full((ragged.M, 32), 7.0, device="cuda")
+    first = ragged.run(ext, B, "balanced", "fp32", chunk_blocks=3).clone()
+    for _ in range(3):
+        ragged.run(ext, B, "balanced", "fp32", chunk_blocks=3, out=out)
+        assert torch.allclose(out, first, rtol=0, atol=1e-6)
+
+
+def test_reference_call_sequence_degree(ext):
+    """The sequence of scripts/DTCSpMM/run_DTC_SpMM.py:45-79 with X = ones:
+    every plan returns C[r, :] = deg(r) exactly."""
+    from dtc.graphs import uniform_rows
+    for name, (rowptr, col, M) in {"fixture18": load_fixture_18()[:3],
+                                   "C1": (lambda a: (a.rowptr, a.col, a.num_rows))(
+                                       uniform_rows(4096, 4096, 16, seed=0, values=None))}.items():
+        dev = "cuda"
+        W = (M + 15) // 16
+        column_index = torch.from_numpy(col).to(dev)
+        row_pointers = torch.from_numpy(rowptr).to(dev)
+        bp = torch.zeros(W, dtype=torch.int32, device=dev)
+        e2c = torch.zeros(col.shape[0], dtype=torch.int32, device=dev)
+        e2r = torch.zeros(col.shape[0], dtype=torch.int32, device=dev)
+        say("reference sequence", name)
+        RowWindowOffset, TCblockRowid, TCblocktileId, TCblockoffset, SparseAToXindex, block_count = \
+            ext.preprocess_gpu(column_index, row_pointers, M, 16, 8, bp, e2c, e2r)
+        deg = torch.from_numpy(np.diff(rowptr).astype(np.float32)).to(dev)
+        for feat in (1, 7, 128, 256):
+            X = torch.ones((M, feat), device=dev)
+            expect = deg[:, None].expand(M, feat)
+            for plan in PLANS:
+                out = ext.run_DTCSpMM(X, RowWindowOffset, TCblocktileId, TCblockoffset, SparseAToXindex, M,
+                                      col.shape[0], plan)
+                assert isinstance(out, list) and len(out) == 1
+                assert torch.equal(out[0], expect), (name, feat, plan)
+            for plan in ("float_nonsplit", "float4_split"):
+                out = ext.run_DTCSpMM_balance(X, TCblockRowid, TCblocktileId, TCblockoffset, SparseAToXindex, M,
+                                              plan)[0]
+                assert torch.equal(out, expect), (name, feat, plan, "balance")
+            out = ext.DTCSpMM_gcn(X, RowWindowOffset, TCblocktileId, TCblockoffset, SparseAToXindex, M,
+                                  col.shape[0])[0]
+            assert torch.equal(out, expect), (name, feat, "gcn")
+            for prec in ("bf16", "fp16", "bf16x3", "bf16x6"):
+                out = ext.run_DTCSpMM(X, RowWindowOffset, TCblocktileId, TCblockoffset, SparseAToXindex, M,
+                                      col.shape[0], "float4_split", precision=prec)[0]
+                assert torch.equal(out, expect), (name, feat, prec)
+
+
+def test_b_row_guard(ext):
+    # Columns at or beyond the B row bound contribute nothing, as the
+    # reference's dense_bound check (DTCSpMM_kernel.cu:2735).
+    rowptr = np.array([0, 2, 3], np.int32)
+    col = np.array([0, 5, 1], np.int32)
+    case = Case(ext, rowptr, col, 2, None, "guard")
+    B = torch.ones((3, 16), device="cuda")  # rows 0..2 only
+    say("spmm B row guard")
+    C = case.run(ext, B, "windows", "fp32")
+    assert torch.equal(C[:, 0].cpu(), torch.tensor([1.0, 1.0]))
