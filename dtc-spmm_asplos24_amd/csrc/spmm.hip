@@ -61,6 +61,21 @@ struct SpmmParams {
   int32_t *queue;      // persistent work counter
 };
 
+// Kernels take the arrays as restrict-qualified arguments: the compiler may
+// then read the wave-uniform ones (block offsets, window ids) through the
+// scalar cache, whose waits do not queue behind the B gathers.
+#define SPMM_KERNEL_ARGS                                                                   \
+  const int32_t *__restrict__ a_rwo, const int32_t *__restrict__ a_rowid,                  \
+      const uint8_t *__restrict__ a_tile, const int32_t *__restrict__ a_off,               \
+      const int32_t *__restrict__ a_atox, const float *__restrict__ a_vals,                \
+      const float *__restrict__ a_B, float *__restrict__ a_C, SpmmParams a_p
+#define SPMM_PARAMS_FROM_ARGS                                                              \
+  SpmmParams {                                                                             \
+    a_rwo, a_rowid, a_tile, a_off, a_atox, a_vals, a_B, a_C, a_p.num_windows,              \
+        a_p.num_blocks, a_p.M, a_p.b_rows, a_p.N, a_p.nnz, a_p.n_ct, a_p.num_items,        \
+        a_p.chunk, a_p.queue                                                               \
+  }
+
 template <int VEC>
 struct VecType {
   typedef float type __attribute__((ext_vector_type(VEC)));
@@ -206,69 +221,134 @@ __device__ __forceinline__ void mfma_group(const float (&a)[8],
   }
 }
 
-// Accumulates the TC blocks [b0, b1) of one window into acc.
+// Metadata of one K=32 group (4 TC blocks), loaded one group ahead of its
+// use so that its latency hides behind the previous group's B gathers.
+struct GroupMeta {
+  int o0, o1, o2, o3, o4;  // element offsets of the group's blocks (wave-uniform)
+  int4 r0, r1;             // SparseAToX of block g (raw; masked where used)
+  int nblk;                // blocks in the group (wave-uniform)
+  int tile;                // tile id of element o0 + lane (raw)
+  float val;               // its value (raw)
+};
+
+// Loads are unconditional and their results are only consumed one group
+// later, so nothing in the current group waits on them.
+__device__ __forceinline__ void load_meta(const SpmmParams &p, int bb, int b1, bool has_vals,
+                                          GroupMeta &m) {
+  const int lane = lane_id();
+  const int g = lane >> 4;
+  const int nblk = min(kGroup, b1 - bb);
+  const int o4 = min(p.off[bb + nblk], p.nnz);
+  m.o0 = min(p.off[bb], o4);
+  m.o1 = min(p.off[bb + min(1, nblk)], o4);
+  m.o2 = min(p.off[bb + min(2, nblk)], o4);
+  m.o3 = min(p.off[bb + min(3, nblk)], o4);
+  m.o4 = o4;
+  m.nblk = nblk;
+  const int blk = min(bb + g, b1 - 1);
+  const int4 *ap = reinterpret_cast<const int4 *>(p.atox + (int64_t)blk * kBlkW);
+  m.r0 = ap[0];
+  m.r1 = ap[1];
+  // The first 64 stored elements of the group, one per lane (a group holds
+  // 36 on average for R-MAT; more than 64 take the slow path in build_a).
+  const int ec = min(m.o0 + lane, p.nnz - 1);
+  m.tile = p.tile[ec];
+  m.val = has_vals ? p.vals[ec] : 1.0f;
+}
+
+// This lane's 8 B rows (-1 for lanes whose block lies past the run).
+__device__ __forceinline__ void meta_rows(const GroupMeta &m, int (&rows)[8]) {
+  const bool gv = (lane_id() >> 4) < m.nblk;
+  rows[0] = gv ? m.r0.x : -1;
+  rows[1] = gv ? m.r0.y : -1;
+  rows[2] = gv ? m.r0.z : -1;
+  rows[3] = gv ? m.r0.w : -1;
+  rows[4] = gv ? m.r1.x : -1;
+  rows[5] = gv ? m.r1.y : -1;
+  rows[6] = gv ? m.r1.z : -1;
+  rows[7] = gv ? m.r1.w : -1;
+}
+
+// B rows of the group into registers: loads from clamped addresses, then
+// zero for slots without a row or columns past N (no branches around loads).
+template <int VEC, int NSUB>
+__device__ __forceinline__ void gather_b(const SpmmParams &p, const int (&rows)[8], int colbase,
+                                         typename VecType<VEC>::type (&breg)[NSUB][8]) {
+  typedef typename VecType<VEC>::type vfloat;
+  const int64_t N = p.N;
+#pragma unroll
+  for (int s = 0; s < NSUB; ++s) {
+    const int col = colbase + s * 16 * VEC;
+    const bool colok = col < p.N;
+    const int colc = colok ? col : 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int r = rows[j];
+      const bool ok = colok && (unsigned)r < (unsigned)p.b_rows;
+      const int rc = ok ? r : 0;
+      const vfloat v = *reinterpret_cast<const vfloat *>(p.B + (int64_t)rc * N + colc);
+      breg[s][j] = ok ? v : vzero<VEC>();
+    }
+  }
+}
+
+// Dense 16x8 A tiles of the group in this wave's LDS, then this lane's
+// operand A_g[q][0..7].
+__device__ __forceinline__ void build_a(const SpmmParams &p, const GroupMeta &m, bool has_vals,
+                                        float *lds, float (&a)[8]) {
+  const int lane = lane_id();
+  const int g = lane >> 4;
+  const int q = lane & 15;
+  floatx4 *t4 = reinterpret_cast<floatx4 *>(lds);
+  const floatx4 z4 = {0.f, 0.f, 0.f, 0.f};
+  t4[lane * 2] = z4;
+  t4[lane * 2 + 1] = z4;
+  wave_lds_sync();
+  if (m.o0 + lane < m.o4) {
+    const int e = m.o0 + lane;
+    const int gg = (e >= m.o1) + (e >= m.o2) + (e >= m.o3);
+    lds[gg * (kBlkH * kBlkW) + (m.tile & (kBlkH * kBlkW - 1))] = m.val;
+  }
+  for (int e = m.o0 + 64 + lane; e < m.o4; e += 64) {
+    const int gg = (e >= m.o1) + (e >= m.o2) + (e >= m.o3);
+    const float v = has_vals ? p.vals[e] : 1.0f;
+    lds[gg * (kBlkH * kBlkW) + (p.tile[e] & (kBlkH * kBlkW - 1))] = v;
+  }
+  wave_lds_sync();
+  const floatx4 lo = t4[(g * (kBlkH * kBlkW) + q * kBlkW) / 4];
+  const floatx4 hi = t4[(g * (kBlkH * kBlkW) + q * kBlkW) / 4 + 1];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    a[j] = lo[j];
+    a[j + 4] = hi[j];
+  }
+  wave_lds_sync();
+}
+
+// Accumulates the TC blocks [b0, b1) of one window into acc.  Software
+// pipelined: group i's B gathers are issued, then group i+1's metadata, then
+// group i's A tile is built and multiplied.
 template <int PREC, int VEC, int NSUB>
 __device__ __forceinline__ void run_blocks(const SpmmParams &p, int b0, int b1, int colbase,
                                            float *lds, floatx4 (&acc)[NSUB][VEC],
                                            bool has_vals) {
   typedef typename VecType<VEC>::type vfloat;
-  const int lane = lane_id();
-  const int g = lane >> 4;
-  const int q = lane & 15;
-  const int64_t N = p.N;
+  if (b0 >= b1) return;
+  GroupMeta cur;
+  load_meta(p, b0, b1, has_vals, cur);
   for (int bb = b0; bb < b1; bb += kGroup) {
-    const int nblk = min(kGroup, b1 - bb);
-    // Gather the 8 B rows of block g (rows of blocks past the run read 0).
-    int rows[8];
-    if (g < nblk) {
-      const int32_t *ap = p.atox + (int64_t)(bb + g) * kBlkW;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) rows[j] = ap[j];
-    } else {
-#pragma unroll
-      for (int j = 0; j < 8; ++j) rows[j] = -1;
-    }
     vfloat breg[NSUB][8];
-#pragma unroll
-    for (int s = 0; s < NSUB; ++s) {
-      const int col = colbase + s * 16 * VEC;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int r = rows[j];
-        if ((unsigned)r < (unsigned)p.b_rows && col < p.N)
-          breg[s][j] = *reinterpret_cast<const vfloat *>(p.B + (int64_t)r * N + col);
-        else
-          breg[s][j] = vzero<VEC>();
-      }
-    }
-    // Dense 16x8 A tiles of the group in this wave's LDS.
-    int o[kGroup + 1];
-#pragma unroll
-    for (int k = 0; k <= kGroup; ++k) o[k] = p.off[bb + min(k, nblk)];
-    floatx4 *t4 = reinterpret_cast<floatx4 *>(lds);
-    const floatx4 z4 = {0.f, 0.f, 0.f, 0.f};
-    t4[lane * 2] = z4;
-    t4[lane * 2 + 1] = z4;
-    wave_lds_sync();
-    const int e_end = min(o[kGroup], p.nnz);
-    for (int e = o[0] + lane; e < e_end; e += 64) {
-      const int gg = (e >= o[1]) + (e >= o[2]) + (e >= o[3]);
-      const float v = has_vals ? p.vals[e] : 1.0f;
-      lds[gg * (kBlkH * kBlkW) + (p.tile[e] & (kBlkH * kBlkW - 1))] = v;
-    }
-    wave_lds_sync();
+    int rows[8];
+    meta_rows(cur, rows);
+    gather_b<VEC, NSUB>(p, rows, colbase, breg);
+    // The last group re-reads its own metadata instead of branching.
+    const int bn = (bb + kGroup < b1) ? bb + kGroup : bb;
+    GroupMeta nxt;
+    load_meta(p, bn, b1, has_vals, nxt);
     float a[8];
-    {
-      const floatx4 lo = t4[(g * (kBlkH * kBlkW) + q * kBlkW) / 4];
-      const floatx4 hi = t4[(g * (kBlkH * kBlkW) + q * kBlkW) / 4 + 1];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        a[j] = lo[j];
-        a[j + 4] = hi[j];
-      }
-    }
-    wave_lds_sync();
+    build_a(p, cur, has_vals, lds, a);
     mfma_group<PREC, VEC, NSUB>(a, breg, acc, has_vals);
+    cur = nxt;
   }
 }
 
@@ -339,19 +419,34 @@ __device__ __forceinline__ void run_unit(const SpmmParams &p, int item, int ct, 
     run_blocks<PREC, VEC, NSUB>(p, b0, b1, colbase, lds, acc, has_vals);
     store_window<VEC, NSUB, false>(p, w, colbase, acc);
   } else {
+    // A window of at most `chunk` blocks belongs whole to the chunk holding
+    // its first block (and may run past that chunk's end); only longer
+    // windows are cut at chunk boundaries and merged with atomic adds into
+    // rows the preparation kernel zeroed.
+    const int T = p.num_blocks;
     const int c0 = item * p.chunk;
-    const int c1 = min(c0 + p.chunk, p.num_blocks);
+    const int c1 = min(c0 + p.chunk, T);
     int b = c0;
     while (b < c1) {
       const int w = wave_uniform(p.rowid[b]);
-      const int se = wave_uniform(segment_end(p.rowid, b, c1, w));
-      zero_acc<VEC, NSUB>(acc);
-      run_blocks<PREC, VEC, NSUB>(p, b, se, colbase, lds, acc, has_vals);
-      const bool start_full = (b == 0) || (p.rowid[b - 1] != w);
-      const bool end_full = (se == p.num_blocks) || (p.rowid[se] != w);
-      if (start_full && end_full) store_window<VEC, NSUB, false>(p, w, colbase, acc);
-      else store_window<VEC, NSUB, true>(p, w, colbase, acc);
-      b = se;
+      const int ws = (b == c0) ? wave_uniform(lower_bound_dev<int32_t>(p.rowid, 0, b, w)) : b;
+      const int we = wave_uniform(max(b + 1, lower_bound_dev<int32_t>(p.rowid, b, T, w + 1)));
+      if (we - ws <= p.chunk) {
+        if (ws < c0) {
+          b = we;
+          continue;
+        }
+        zero_acc<VEC, NSUB>(acc);
+        run_blocks<PREC, VEC, NSUB>(p, b, we, colbase, lds, acc, has_vals);
+        store_window<VEC, NSUB, false>(p, w, colbase, acc);
+        b = we;
+      } else {
+        const int se = min(we, c1);
+        zero_acc<VEC, NSUB>(acc);
+        run_blocks<PREC, VEC, NSUB>(p, b, se, colbase, lds, acc, has_vals);
+        store_window<VEC, NSUB, true>(p, w, colbase, acc);
+        b = se;
+      }
     }
   }
 }
@@ -360,7 +455,8 @@ __device__ __forceinline__ void run_unit(const SpmmParams &p, int item, int ct, 
 // 8 XCDs round robin, so when n_ct divides 8 every XCD (and its L2) only
 // sees one column tile of B: 1/n_ct of each gathered row.
 template <int PREC, int VEC, int NSUB, bool BALANCED>
-__global__ __launch_bounds__(256) void spmm_kernel(SpmmParams p) {
+__global__ __launch_bounds__(256) void spmm_kernel(SPMM_KERNEL_ARGS) {
+  const SpmmParams p = SPMM_PARAMS_FROM_ARGS;
   __shared__ __attribute__((aligned(16))) float s_tile[kWavesPerBlock][kTileFloats];
   const int wave = wave_uniform(threadIdx.x >> 6);
   const int ct = blockIdx.x % p.n_ct;
@@ -372,7 +468,8 @@ __global__ __launch_bounds__(256) void spmm_kernel(SpmmParams p) {
 // Persistent variant of the balanced schedule: every wave pulls units from a
 // global counter until the list is exhausted.
 template <int PREC, int VEC, int NSUB>
-__global__ __launch_bounds__(256) void spmm_kernel_persistent(SpmmParams p) {
+__global__ __launch_bounds__(256) void spmm_kernel_persistent(SPMM_KERNEL_ARGS) {
+  const SpmmParams p = SPMM_PARAMS_FROM_ARGS;
   __shared__ __attribute__((aligned(16))) float s_tile[kWavesPerBlock][kTileFloats];
   const int wave = wave_uniform(threadIdx.x >> 6);
   const int lane = lane_id();
@@ -388,8 +485,8 @@ __global__ __launch_bounds__(256) void spmm_kernel_persistent(SpmmParams p) {
 }
 
 // Balanced-schedule preparation, one wave per window: zero the rows of
-// windows that no chunk stores whole (empty windows, windows spanning
-// several chunks) and reset the work counter.
+// windows no chunk stores whole (empty windows and windows longer than a
+// chunk) and reset the work counter.
 __global__ __launch_bounds__(256) void k_balance_prep(const int32_t *rowid, int32_t T,
                                                       int32_t W, int32_t M, int32_t N,
                                                       int32_t chunk, float *C, int32_t *queue) {
@@ -399,18 +496,24 @@ __global__ __launch_bounds__(256) void k_balance_prep(const int32_t *rowid, int3
   if (w >= W) return;
   const int fb = lower_bound_dev<int32_t>(rowid, 0, T, (int32_t)w);
   const int lb = lower_bound_dev<int32_t>(rowid, fb, T, (int32_t)(w + 1));
-  const bool zero = (fb == lb) || (fb / chunk != (lb - 1) / chunk);
-  if (!zero) return;
+  if (fb < lb && lb - fb <= chunk) return;
   const int64_t r0 = w * kBlkH;
   const int64_t r1 = (r0 + kBlkH < (int64_t)M) ? r0 + kBlkH : (int64_t)M;
   float *base = C + r0 * (int64_t)N;
   const int64_t count = (r1 - r0) * (int64_t)N;
-  for (int64_t idx = lane; idx < count; idx += 64) base[idx] = 0.f;
+  if ((N & 3) == 0 && ((uintptr_t)base & 15) == 0) {
+    floatx4 *b4 = reinterpret_cast<floatx4 *>(base);
+    const floatx4 z = {0.f, 0.f, 0.f, 0.f};
+    for (int64_t idx = lane; idx < count / 4; idx += 64) b4[idx] = z;
+  } else {
+    for (int64_t idx = lane; idx < count; idx += 64) base[idx] = 0.f;
+  }
 }
 
 // ------------------------------------------------------------ host side
 
-typedef void (*KernelFn)(SpmmParams);
+typedef void (*KernelFn)(const int32_t *, const int32_t *, const uint8_t *, const int32_t *,
+                         const int32_t *, const float *, const float *, float *, SpmmParams);
 
 struct Variant {
   int vec;
@@ -466,20 +569,41 @@ dtc_status_t resolve_shape(const dtc_spmm_options_t *opt, int32_t n, const float
 }
 
 dtc_status_t check_common(const uint8_t *tile, const int32_t *off, const int32_t *atox,
-                          int32_t num_nodes, const float *b, int32_t b_rows, int32_t n, float *c,
-                          const dtc_spmm_options_t *opt) {
-  if (num_nodes < 0 || b_rows < 0) return fail(DTC_ERROR_INVALID_VALUE, "sizes must be >= 0");
+                          int32_t num_blocks, int32_t nnz, int32_t num_nodes, const float *b,
+                          int32_t b_rows, int32_t n, float *c, const dtc_spmm_options_t *opt) {
+  if (num_nodes < 0 || b_rows < 0 || num_blocks < 0 || nnz < 0)
+    return fail(DTC_ERROR_INVALID_VALUE, "sizes must be >= 0");
   if (n <= 0) return fail(DTC_ERROR_INVALID_VALUE, "N (feature width) must be >= 1");
   if (!off) return fail(DTC_ERROR_INVALID_VALUE, "TCblock_offset is NULL");
   if (num_nodes > 0 && !c) return fail(DTC_ERROR_INVALID_VALUE, "C is NULL");
   if (b_rows > 0 && !b) return fail(DTC_ERROR_INVALID_VALUE, "B is NULL");
-  (void)tile;
-  (void)atox;
+  if (num_blocks > 0) {
+    if (!tile || !atox)
+      return fail(DTC_ERROR_INVALID_VALUE, "TCblocktile_id and sparse_AToX_idx are required");
+    if (nnz < 1) return fail(DTC_ERROR_INVALID_VALUE, "TC blocks without stored elements");
+    if ((uintptr_t)atox % 16 != 0)
+      return fail(DTC_ERROR_INVALID_VALUE, "sparse_AToX_idx must be 16-byte aligned");
+  }
   const int prec = opt ? opt->precision : DTC_PRECISION_FP32;
   if (prec < DTC_PRECISION_FP32 || prec > DTC_PRECISION_BF16X6)
     return fail(DTC_ERROR_INVALID_VALUE, "unknown precision");
   if ((int64_t)num_nodes * n > ((int64_t)1 << 40))
     return fail(DTC_ERROR_UNSUPPORTED, "output too large");
+  return DTC_SUCCESS;
+}
+
+__global__ __launch_bounds__(256) void k_zero(float *c, int64_t count) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) c[i] = 0.f;
+}
+
+// C = 0 when B has no rows (every column of A is out of range).
+dtc_status_t zero_output(float *c, int32_t m, int32_t n, hipStream_t stream) {
+  const int64_t count = (int64_t)m * n;
+  if (count == 0) return DTC_SUCCESS;
+  const unsigned grid = (unsigned)std::min<int64_t>(65536, ceil_div(count, 256));
+  k_zero<<<grid, 256, 0, stream>>>(c, count);
+  DTC_LAUNCH_CHECK("zero_output");
   return DTC_SUCCESS;
 }
 
@@ -499,8 +623,8 @@ dtc_status_t dtc_spmm_windows(const int32_t *row_window_offset, const uint8_t *t
                               dtc_stream_t stream_) {
   clear_error();
   hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
-  dtc_status_t st = check_common(tcblocktile_id, tcblock_offset, sparse_atox, num_nodes, b,
-                                 b_rows, n, c, options);
+  dtc_status_t st = check_common(tcblocktile_id, tcblock_offset, sparse_atox, num_blocks, nnz,
+                                 num_nodes, b, b_rows, n, c, options);
   if (st != DTC_SUCCESS) return st;
   if (!row_window_offset) return fail(DTC_ERROR_INVALID_VALUE, "Rowwindow_offset is NULL");
   if (num_windows < 0 || num_blocks < 0 || nnz < 0)
@@ -516,6 +640,7 @@ dtc_status_t dtc_spmm_windows(const int32_t *row_window_offset, const uint8_t *t
   // Only windows that hold output rows are processed.
   const int32_t W = (int32_t)std::min<int64_t>(num_windows, ceil_div(num_nodes, kBlkH));
   if (W == 0) return DTC_SUCCESS;
+  if (b_rows == 0) return zero_output(c, num_nodes, n, stream);
   SpmmParams p{};
   p.rwo = row_window_offset;
   p.tile = tcblocktile_id;
@@ -536,7 +661,8 @@ dtc_status_t dtc_spmm_windows(const int32_t *row_window_offset, const uint8_t *t
   KernelFn fn = pick_kernel(prec, var.vec, var.nsub, 0);
   const int64_t grid = (int64_t)p.n_ct * ceil_div(W, kWavesPerBlock);
   if (grid > INT32_MAX) return fail(DTC_ERROR_UNSUPPORTED, "grid too large");
-  hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(256), 0, stream, p);
+  hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(256), 0, stream, p.rwo, p.rowid, p.tile, p.off,
+                     p.atox, p.vals, p.B, p.C, p);
   DTC_LAUNCH_CHECK("spmm_windows");
   return DTC_SUCCESS;
 }
@@ -557,10 +683,9 @@ dtc_status_t dtc_spmm_balanced(const int32_t *tcblock_rowid, const uint8_t *tcbl
                                size_t workspace_bytes, dtc_stream_t stream_) {
   clear_error();
   hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
-  dtc_status_t st = check_common(tcblocktile_id, tcblock_offset, sparse_atox, num_nodes, b,
-                                 b_rows, n, c, options);
+  dtc_status_t st = check_common(tcblocktile_id, tcblock_offset, sparse_atox, num_blocks, nnz,
+                                 num_nodes, b, b_rows, n, c, options);
   if (st != DTC_SUCCESS) return st;
-  if (num_blocks < 0 || nnz < 0) return fail(DTC_ERROR_INVALID_VALUE, "num_blocks and nnz must be >= 0");
   if (num_blocks > 0 && (!tcblock_rowid || !tcblocktile_id || !sparse_atox))
     return fail(DTC_ERROR_INVALID_VALUE, "TCblock_rowid, TCblocktile_id and sparse_AToX_idx are required");
   Variant var;
@@ -572,6 +697,7 @@ dtc_status_t dtc_spmm_balanced(const int32_t *tcblock_rowid, const uint8_t *tcbl
     return fail(DTC_ERROR_WORKSPACE, "persistent balanced SpMM needs a 256-byte workspace");
   const int32_t W = (int32_t)ceil_div(num_nodes, kBlkH);
   if (W == 0) return DTC_SUCCESS;
+  if (b_rows == 0) return zero_output(c, num_nodes, n, stream);
   SpmmParams p{};
   p.rowid = tcblock_rowid;
   p.tile = tcblocktile_id;
@@ -599,7 +725,8 @@ dtc_status_t dtc_spmm_balanced(const int32_t *tcblock_rowid, const uint8_t *tcbl
     KernelFn fn = pick_kernel(prec, var.vec, var.nsub, 1);
     const int64_t grid = (int64_t)p.n_ct * ceil_div(p.num_items, kWavesPerBlock);
     if (grid > INT32_MAX) return fail(DTC_ERROR_UNSUPPORTED, "grid too large");
-    hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(256), 0, stream, p);
+    hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(256), 0, stream, p.rwo, p.rowid, p.tile, p.off,
+                     p.atox, p.vals, p.B, p.C, p);
   } else {
     int dev = 0, cus = 0;
     DTC_HIP_CHECK(hipGetDevice(&dev));
@@ -608,7 +735,8 @@ dtc_status_t dtc_spmm_balanced(const int32_t *tcblock_rowid, const uint8_t *tcbl
     const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(ceil_div(units, kWavesPerBlock),
                                                                 (int64_t)cus * 8));
     KernelFn fn = pick_kernel(prec, var.vec, var.nsub, 2);
-    hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(256), 0, stream, p);
+    hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(256), 0, stream, p.rwo, p.rowid, p.tile, p.off,
+                     p.atox, p.vals, p.B, p.C, p);
   }
   DTC_LAUNCH_CHECK("spmm_balanced");
   return DTC_SUCCESS;

@@ -144,14 +144,19 @@ def test_schedules_agree_rmat(ext, rmat12):
 def test_repeated_launches_identical(ext, ragged):
     # The reference's balanced op accumulates into its output on every one of
     # its 1000 timed launches; every launch here starts from the zeroed rows.
+    # Windows merged from three or more chunks add in a run-dependent order,
+    # so repeated results agree to float32 rounding, not bitwise.
     rng = np.random.default_rng(8)
-    B = torch.from_numpy(rng.standard_normal((ragged.M, 32)).astype(np.float32)).cuda()
+    Bn = rng.standard_normal((ragged.M, 32)).astype(np.float32)
+    B = torch.from_numpy(Bn).cuda()
+    C64, Cabs = ragged.ref(Bn)
     say("spmm repeat")
     out = torch.full((ragged.M, 32), 7.0, device="cuda")
     first = ragged.run(ext, B, "balanced", "fp32", chunk_blocks=3).clone()
     for _ in range(3):
         ragged.run(ext, B, "balanced", "fp32", chunk_blocks=3, out=out)
-        assert torch.allclose(out, first, rtol=0, atol=1e-6)
+        assert torch.allclose(out, first, rtol=1e-5, atol=1e-5)
+        assert componentwise(out.cpu().numpy(), C64, Cabs) <= TOL["fp32"]
 
 
 def test_reference_call_sequence_degree(ext):
