@@ -7,20 +7,19 @@ float32; B = 1,048,576 x 256 float32 N(0, 1).
 
 A step is one SpMM launch C = A * B over the resident ME-TCF (built once on
 the GPU and timed separately, as the reference times preprocess and SpMM
-apart: DTCSpMM.cpp:337-348 vs DTCSpMM_kernel.cu:841-856).  The variant is
-chosen by exhaustive timing over schedule x precision x chunk among the
-variants whose componentwise error against the float64 CSR product is
-<= 1e-5; the plain FP32 variant is timed the same way for comparison.
+apart: DTCSpMM.cpp:337-348 vs DTCSpMM_kernel.cu:841-856).  The variant is the
+learned selector's pick (dtc/selector.py, weights in dtc/data/); every
+variant is also timed (exhaustive timing: the selector's judge and fallback)
+and its componentwise error against the float64 CSR product is recorded.
+The plain-FP32 path is the fastest FP32 variant of that table.
 
-Prints one JSON line (see the keys below).  --gpus N > 1 exits: this build
-runs on one GPU.
+Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
-import platform
 import sys
 import time
 
@@ -30,6 +29,7 @@ sys.path[:0] = [PKG]
 
 ERR_TARGET = 1e-5
 HBM_PEAK_GBS = 8000.0
+EXTRA_PRECISIONS = ["bf16x3", "fp16", "bf16"]  # timed and reported, fail the error gate
 
 
 def log(*a):
@@ -48,6 +48,29 @@ def timed(stream, fn, iters):
     return e0.elapsed_time(e1) / iters
 
 
+def copy_probe():
+    """Three copies of a 1 GiB float32 tensor: each reads and writes exactly
+    2**30 bytes, the reference point for FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    src = torch.ones(1 << 28, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": 1 << 30, "launches": 3}), flush=True)
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -55,9 +78,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--variant", default="auto",
-                    help="auto (exhaustive choice) or schedule:precision:chunk, e.g. balanced:fp32:64")
+    ap.add_argument("--variant", default="selector",
+                    help="selector (learned pick), exhaustive, or schedule:precision:chunk:vec:subtiles")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-table", action="store_true", help="skip the exhaustive variant table")
     ap.add_argument("--traffic-file", default=os.path.join(REPO, "profiles", "traffic_rmat20_n256.json"))
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy (counter calibration)")
@@ -67,12 +91,13 @@ def main():
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this DTC-SpMM build runs on one GPU only; multi-GPU runs are not part of it "
-              "(see DESIGN.md for how the path would shard).", file=sys.stderr)
+              "(DESIGN.md describes how the path would shard).", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
     import torch
 
+    from dtc import selector as sel
     from dtc.graphs import rmat
     from dtc.matrix import DTCMatrix
     from dtc.metrics import compulsory_bytes, gather_model_bytes, spmm_flops
@@ -91,7 +116,6 @@ def main():
     col_d = torch.from_numpy(a.col).to(dev)
     vals_d = torch.from_numpy(a.values).to(dev)
     A = DTCMatrix(rowptr_d, col_d, vals_d, num_rows=M, num_cols=M)  # warm-up build
-    torch.cuda.synchronize()
     pre_ms = []
     for _ in range(3):
         torch.cuda.synchronize()
@@ -124,49 +148,52 @@ def main():
         norm = float(d.max() / np.abs(C64).max())
         return comp, norm
 
-    def runner(sched, prec, chunk, persistent=False, vec=0, sub=0):
-        return lambda: A.spmm(B, schedule=sched, precision=prec, chunk_blocks=chunk,
-                              persistent=persistent, vec=vec, subtiles=sub, out=C)
+    def runner(v):
+        sched, prec, chunk, vec, sub, pers = v
+        return lambda: A.spmm(B, schedule=sched, precision=prec, chunk_blocks=chunk, vec=vec,
+                              subtiles=sub, persistent=pers, out=C)
 
-    # ---- exhaustive variant timing (the selector's fallback and judge)
-    candidates = []
-    if args.variant == "auto":
-        for prec in ["fp32", "bf16x6", "bf16x3", "fp16", "bf16"]:
-            chunks = [128, 256, 512, 1024] if prec == "fp32" else [256, 512]
-            candidates.append(("windows", prec, 0, False, 4, 2))
-            for chunk in chunks:
-                for vec, sub in [(4, 2), (4, 1)]:
-                    candidates.append(("balanced", prec, chunk, False, vec, sub))
-            candidates.append(("balanced", prec, 256, True, 4, 2))
-    else:
-        parts = args.variant.split(":")
-        s_, p_, c_ = parts[:3]
-        vec, sub = (int(parts[3]), int(parts[4])) if len(parts) >= 5 else (0, 0)
-        candidates.append((s_, p_, int(c_), False, vec, sub))
+    # ---- the learned selector's pick
+    feats = sel.features(A, N)
+    pick = None
+    if os.path.exists(sel.DEFAULT_MODEL):
+        model = sel.MLPSelector.load(sel.DEFAULT_MODEL)
+        pick = int(model.choose(feats[None, :])[0])
+        log(f"selector pick: {sel.variant_name(sel.VARIANTS[pick])}")
+
+    # ---- exhaustive table (the selector's judge and fallback)
+    table_variants = list(sel.VARIANTS)
+    if not args.no_table:
+        table_variants += [("balanced", p, 256, 4, 1, False) for p in EXTRA_PRECISIONS]
     table = []
     err_cache = {}
-    for sched, prec, chunk, pers, vec, sub in candidates:
-        fn = runner(sched, prec, chunk, pers, vec, sub)
-        fn()
-        torch.cuda.synchronize()
-        key = prec
-        if key not in err_cache:
-            err_cache[key] = errors(C)
-        ms = timed(stream, fn, 5)
-        table.append(dict(schedule=sched, precision=prec, chunk=chunk, persistent=pers, vec=vec,
-                          subtiles=sub, ms=ms,
-                          gflops=spmm_flops(nnz, N) / ms / 1e6, err=err_cache[key][0],
-                          err_norm=err_cache[key][1]))
-        log(f"  {sched:9s} {prec:7s} chunk={chunk:4d} pers={int(pers)} vec={vec} sub={sub} {ms:8.3f} ms "
-            f"err={err_cache[key][0]:.2e}")
-    fp32_rows = [r for r in table if r["precision"] == "fp32"]
-    best_fp32 = min(fp32_rows, key=lambda r: r["ms"]) if fp32_rows else None
-    ok = [r for r in table if r["err"] <= ERR_TARGET]
-    chosen = min(ok, key=lambda r: r["ms"]) if ok else min(table, key=lambda r: r["ms"])
+    if not args.no_table:
+        for v in table_variants:
+            fn = runner(v)
+            fn()
+            torch.cuda.synchronize()
+            if v[1] not in err_cache:
+                err_cache[v[1]] = errors(C)
+            ms = timed(stream, fn, 5)
+            table.append(dict(variant=sel.variant_name(v), precision=v[1], ms=ms,
+                              gflops=spmm_flops(nnz, N) / ms / 1e6, err=err_cache[v[1]][0],
+                              err_norm=err_cache[v[1]][1]))
+            log(f"  {sel.variant_name(v):42s} {ms:8.3f} ms  err={err_cache[v[1]][0]:.2e}")
+
+    if args.variant == "selector" and pick is not None:
+        chosen_v = sel.VARIANTS[pick]
+        how = "selector"
+    elif args.variant in ("selector", "exhaustive"):
+        ok = [(r["ms"], i) for i, r in enumerate(table) if r["err"] <= ERR_TARGET]
+        chosen_v = table_variants[min(ok)[1]]
+        how = "exhaustive"
+    else:
+        s_, p_, c_, vec_, sub_ = args.variant.split(":")
+        chosen_v = (s_, p_, int(c_), int(vec_), int(sub_), False)
+        how = "fixed"
 
     # ---- timed region: K steps of the chosen variant
-    fn = runner(chosen["schedule"], chosen["precision"], chosen["chunk"], chosen["persistent"],
-                chosen["vec"], chosen["subtiles"])
+    fn = runner(chosen_v)
     for _ in range(args.warmup):
         fn()
     torch.cuda.synchronize()
@@ -183,13 +210,24 @@ def main():
         with open(args.traffic_file) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
 
-    fp32_ms = None
-    if best_fp32 is not None:
-        fr = runner(best_fp32["schedule"], "fp32", best_fp32["chunk"], best_fp32["persistent"],
-                    best_fp32["vec"], best_fp32["subtiles"])
+    # ---- plain FP32 path: fastest FP32 variant, timed like the chosen one
+    fp32 = None
+    fp32_rows = [(r["ms"], i) for i, r in enumerate(table) if r["precision"] == "fp32"]
+    if fp32_rows:
+        v32 = table_variants[min(fp32_rows)[1]]
+        fr = runner(v32)
         for _ in range(3):
             fr()
         fp32_ms = timed(stream, fr, args.steps)
+        fp32 = {"variant": sel.variant_name(v32), "ms": round(fp32_ms, 4),
+                "gflops": round(flops / fp32_ms / 1e6, 2), "speedup_of_chosen": round(fp32_ms / ms, 3)}
+
+    selector_info = None
+    if table:
+        best = min(range(len(sel.VARIANTS)), key=lambda i: table[i]["ms"])
+        selector_info = {"pick": None if pick is None else sel.variant_name(sel.VARIANTS[pick]),
+                         "exhaustive_best": sel.variant_name(sel.VARIANTS[best]),
+                         "regret": None if pick is None else round(table[pick]["ms"] / table[best]["ms"] - 1, 4)}
 
     # ---- CPU baseline: the oracle's ME-TCF SpMM port on the host cores
     cpu = None
@@ -213,12 +251,13 @@ def main():
             t = time.perf_counter()
             oracle.metcf(a.rowptr, a.col, M, values=a.values, nthreads=cores)
             tb.append(time.perf_counter() - t)
-        cpu = {"value": flops / t_cpu / 1e9, "unit": "GFLOP/s", "cores": cores, "kind": "port",
+        cpu = {"value": round(flops / t_cpu / 1e9, 2), "unit": "GFLOP/s", "cores": cores, "kind": "port",
                "sample": f"full SpMM of this workload (R-MAT scale {args.scale}, N={N}) with the CPU "
                          f"oracle's float32 ME-TCF port, median of 3 runs; host ME-TCF build "
-                         f"{np.median(tb) * 1e3:.0f} ms",
-               "cpu_model": platform.processor() or _cpu_model()}
+                         f"{np.median(tb) * 1e3:.0f} ms", "cpu_model": _cpu_model()}
 
+    dtypes = {"fp32": "f32", "bf16x6": "bf16x6 (f32 accumulate)", "bf16x3": "bf16x3 (f32 accumulate)",
+              "fp16": "f16 (f32 accumulate)", "bf16": "bf16 (f32 accumulate)"}
     result = {
         "metric": "SpMM GFLOP/s (2*nnz*N) + % HBM roofline, mixed vs FP32 at <=1e-5 rel err",
         "value": round(gflops, 2),
@@ -230,24 +269,19 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp32": "f32", "bf16x6": "bf16x6(f32 acc)", "bf16x3": "bf16x3(f32 acc)",
-                  "fp16": "f16(f32 acc)", "bf16": "bf16(f32 acc)"}[chosen["precision"]],
+        "dtype": dtypes[chosen_v[1]],
         "data": "synthetic (seeded R-MAT graph, B from torch N(0,1) seed 2)",
         "config": {"workload": f"R-MAT scale {args.scale} (a=.57,b=.19,c=.19), edge factor 16, "
                                f"symmetrised, 1/sqrt(deg_i deg_j) values; B {M}x{N} f32",
                    "num_rows": M, "nnz": nnz, "N": N, "tc_blocks": T, "row_windows": W,
-                   "sum_unique_columns": sum_u,
-                   "variant": {k: chosen[k] for k in ("schedule", "precision", "chunk", "persistent",
-                                                      "vec", "subtiles")},
+                   "sum_unique_columns": sum_u, "variant": sel.variant_name(chosen_v), "chosen_by": how,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,
         "error_vs_fp64": {"componentwise": comp_err, "normwise": norm_err, "target": ERR_TARGET},
-        "fp32_path": None if fp32_ms is None else {
-            "ms": round(fp32_ms, 4), "gflops": round(flops / fp32_ms / 1e6, 2),
-            "variant": {k: best_fp32[k] for k in ("schedule", "chunk", "persistent", "vec", "subtiles")},
-            "speedup_of_chosen": round(fp32_ms / ms, 3)},
+        "fp32_path": fp32,
+        "selector": selector_info,
         "preprocess_ms": round(float(np.median(pre_ms)), 3),
         "bytes_model_per_launch": bytes_model,
         "compulsory_bytes": compulsory_bytes(M, M, N, nnz),
@@ -255,29 +289,6 @@ def main():
                       for k, v in r.items()} for r in table],
     }
     print(json.dumps(result), flush=True)
-
-
-def copy_probe():
-    """Three copies of a 1 GiB float32 tensor: each reads and writes exactly
-    2**30 bytes, the reference point for FETCH_SIZE / WRITE_SIZE."""
-    import torch
-    src = torch.ones(1 << 28, device="cuda")
-    dst = torch.empty_like(src)
-    for _ in range(3):
-        dst.copy_(src)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_probe_bytes": 1 << 30, "launches": 3}), flush=True)
-
-
-def _cpu_model():
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
-    except OSError:
-        pass
-    return "unknown"
 
 
 if __name__ == "__main__":

@@ -443,9 +443,91 @@ Tensor gather(Tensor values, Tensor index) {
   return out;
 }
 
+// ------------------------------------------------------------- selector
+
+void check_f32_2d(const Tensor &t, const char *name) {
+  TORCH_CHECK(t.is_cuda() && t.is_contiguous() && t.scalar_type() == torch::kFloat32 && t.dim() == 2,
+              name, " must be a contiguous 2-D float32 CUDA tensor");
+}
+
+Tensor mlp_linear_forward(Tensor x, Tensor w, c10::optional<Tensor> bias, bool relu) {
+  check_f32_2d(x, "x");
+  check_f32_2d(w, "w");
+  TORCH_CHECK(x.size(1) == w.size(0), "x and w shapes do not chain");
+  if (bias.has_value()) {
+    CHECK_INPUT(*bias);
+    TORCH_CHECK(bias->numel() == w.size(1), "bias must have out_dim entries");
+  }
+  const c10::OptionalDeviceGuard guard(x.device());
+  Tensor y = torch::empty({x.size(0), w.size(1)}, x.options());
+  check_status(dtc_mlp_linear_forward(x.data_ptr<float>(), w.data_ptr<float>(),
+                                      bias.has_value() ? bias->data_ptr<float>() : nullptr, y.data_ptr<float>(),
+                                      (int32_t)x.size(0), (int32_t)x.size(1), (int32_t)w.size(1), relu ? 1 : 0,
+                                      current_stream()),
+               "mlp_linear_forward");
+  return y;
+}
+
+py::tuple mlp_linear_backward(Tensor x, Tensor w, Tensor y, Tensor dy, bool relu, bool need_dx) {
+  check_f32_2d(x, "x");
+  check_f32_2d(w, "w");
+  check_f32_2d(y, "y");
+  check_f32_2d(dy, "dy");
+  TORCH_CHECK(y.sizes() == dy.sizes() && y.size(0) == x.size(0) && y.size(1) == w.size(1) && x.size(1) == w.size(0),
+              "inconsistent shapes");
+  const c10::OptionalDeviceGuard guard(x.device());
+  Tensor dw = torch::empty_like(w);
+  Tensor db = torch::empty({w.size(1)}, w.options());
+  Tensor dx;
+  if (need_dx) dx = torch::empty_like(x);
+  check_status(dtc_mlp_linear_backward(x.data_ptr<float>(), w.data_ptr<float>(), y.data_ptr<float>(),
+                                       dy.data_ptr<float>(), need_dx ? dx.data_ptr<float>() : nullptr,
+                                       dw.data_ptr<float>(), db.data_ptr<float>(), (int32_t)x.size(0),
+                                       (int32_t)x.size(1), (int32_t)w.size(1), relu ? 1 : 0, current_stream()),
+               "mlp_linear_backward");
+  return py::make_tuple(need_dx ? py::cast(dx) : py::none(), dw, db);
+}
+
+py::tuple mlp_mse(Tensor pred, Tensor target, Tensor mask, double inv_count) {
+  check_f32_2d(pred, "pred");
+  check_f32_2d(target, "target");
+  check_f32_2d(mask, "mask");
+  TORCH_CHECK(pred.sizes() == target.sizes() && pred.sizes() == mask.sizes(), "shapes differ");
+  const c10::OptionalDeviceGuard guard(pred.device());
+  Tensor d = torch::empty_like(pred);
+  Tensor loss = torch::empty({1}, pred.options());
+  check_status(dtc_mlp_mse(pred.data_ptr<float>(), target.data_ptr<float>(), mask.data_ptr<float>(), pred.numel(),
+                           (float)inv_count, d.data_ptr<float>(), loss.data_ptr<float>(), current_stream()),
+               "mlp_mse");
+  return py::make_tuple(d, loss);
+}
+
+void adam_step(Tensor param, Tensor grad, Tensor m, Tensor v, int64_t step, double lr, double beta1, double beta2,
+               double eps) {
+  for (const Tensor *t : {&param, &grad, &m, &v}) {
+    CHECK_INPUT(*t);
+    CHECK_DTYPE(*t, torch::kFloat32);
+    TORCH_CHECK(t->numel() == param.numel(), "Adam tensors must have the parameter's size");
+  }
+  const c10::OptionalDeviceGuard guard(param.device());
+  check_status(dtc_adam_step(param.data_ptr<float>(), grad.data_ptr<float>(), m.data_ptr<float>(),
+                             v.data_ptr<float>(), param.numel(), (float)lr, (float)beta1, (float)beta2, (float)eps,
+                             (int32_t)step, current_stream()),
+               "adam_step");
+}
+
 }  // namespace
 
 PYBIND11_MODULE(DTCSpMM, m) {
+  m.def("mlp_linear_forward", &mlp_linear_forward, "y = act(x W + b)", py::arg("x"), py::arg("w"),
+        py::arg("bias") = py::none(), py::arg("relu") = false);
+  m.def("mlp_linear_backward", &mlp_linear_backward, "(dx, dw, db) of one layer", py::arg("x"), py::arg("w"),
+        py::arg("y"), py::arg("dy"), py::arg("relu") = false, py::arg("need_dx") = true);
+  m.def("mlp_mse", &mlp_mse, "masked MSE -> (dpred, loss)", py::arg("pred"), py::arg("target"), py::arg("mask"),
+        py::arg("inv_count"));
+  m.def("adam_step", &adam_step, "in-place Adam update", py::arg("param"), py::arg("grad"), py::arg("m"),
+        py::arg("v"), py::arg("step"), py::arg("lr"), py::arg("beta1") = 0.9, py::arg("beta2") = 0.999,
+        py::arg("eps") = 1e-8);
   m.doc() = "DTC-SpMM for AMD MI355X (gfx950): ME-TCF build and SpMM over libdtc";
   m.def("preprocess_gpu", &preprocess_gpu, "ME-TCF build on the GPU", py::arg("edgeList"), py::arg("nodePointer"),
         py::arg("num_nodes"), py::arg("blockSize_h"), py::arg("blockSize_w"), py::arg("blockPartition"),

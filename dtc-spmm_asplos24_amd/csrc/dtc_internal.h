@@ -122,6 +122,30 @@ __device__ __forceinline__ int lower_bound_dev(const T *a, int lo, int hi, T key
   return lo;
 }
 
+// First index in [lo, hi) with a[idx] >= key (a ascending), computed by the
+// whole wavefront: each round probes 64 evenly spaced positions and keeps the
+// one interval the ballot points at, so a search over n elements takes
+// ceil(log64 n) rounds of one coalesced load instead of log2 n dependent
+// loads.  Must be called by all 64 lanes with the same arguments.
+template <class T>
+__device__ __forceinline__ int wave_lower_bound(const T *a, int lo, int hi, T key) {
+  const int lane = lane_id();
+  while (hi - lo > kWave) {
+    const int step = (hi - lo + kWave - 1) / kWave;
+    const int p = lo + lane * step;
+    const bool less = p < hi && a[p] < key;
+    const int k = __popcll(__ballot(less));
+    if (k == 0) return lo;
+    const int nlo = lo + (k - 1) * step + 1;
+    const int nhi = min(hi, lo + k * step);
+    lo = nlo;
+    hi = nhi;
+  }
+  const int p = lo + lane;
+  const bool less = p < hi && a[p] < key;
+  return lo + __popcll(__ballot(less));
+}
+
 // ----------------------------------------------------- device-wide pieces
 // (implemented in primitives.hip)
 

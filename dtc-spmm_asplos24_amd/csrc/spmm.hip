@@ -392,16 +392,22 @@ __device__ __forceinline__ void store_window(const SpmmParams &p, int w, int col
   }
 }
 
-// First block index in [b, c1) whose window differs from w (c1 if none).
-__device__ __forceinline__ int segment_end(const int32_t *rowid, int b, int c1, int w) {
-  const int lane = lane_id();
-  for (int base = b; base < c1; base += 64) {
-    const int idx = base + lane;
-    const int rw = idx < c1 ? rowid[idx] : -1;
-    const uint64_t m = __ballot(rw != w);
-    if (m) return base + (int)__builtin_ctzll(m);
-  }
-  return c1;
+// End of window w whose block b is known: one ballot over the next 64 block
+// ids covers most windows; longer ones finish with a wave-parallel search.
+__device__ __forceinline__ int window_end_from(const int32_t *rowid, int b, int T, int w) {
+  const int idx = b + lane_id();
+  const uint64_t m = __ballot(idx >= T || rowid[idx] != w);
+  if (m) return b + (int)__builtin_ctzll(m);
+  return wave_lower_bound<int32_t>(rowid, b + 64, T, w + 1);
+}
+
+// Start of window w whose block b is known (the mirror image).
+__device__ __forceinline__ int window_start_from(const int32_t *rowid, int b, int w) {
+  if (b == 0) return 0;
+  const int idx = b - 1 - lane_id();  // b-1, b-2, ..., b-64
+  const uint64_t m = __ballot(idx < 0 || rowid[idx] != w);
+  if (m) return b - (int)__builtin_ctzll(m);
+  return wave_lower_bound<int32_t>(rowid, 0, b - 64, w);
 }
 
 template <int PREC, int VEC, int NSUB, bool BALANCED>
@@ -429,8 +435,8 @@ __device__ __forceinline__ void run_unit(const SpmmParams &p, int item, int ct, 
     int b = c0;
     while (b < c1) {
       const int w = wave_uniform(p.rowid[b]);
-      const int ws = (b == c0) ? wave_uniform(lower_bound_dev<int32_t>(p.rowid, 0, b, w)) : b;
-      const int we = wave_uniform(max(b + 1, lower_bound_dev<int32_t>(p.rowid, b, T, w + 1)));
+      const int ws = (b == c0) ? wave_uniform(window_start_from(p.rowid, b, w)) : b;
+      const int we = wave_uniform(max(b + 1, window_end_from(p.rowid, b, T, w)));
       if (we - ws <= p.chunk) {
         if (ws < c0) {
           b = we;
@@ -494,8 +500,8 @@ __global__ __launch_bounds__(256) void k_balance_prep(const int32_t *rowid, int3
   const int lane = threadIdx.x & 63;
   const int64_t w = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   if (w >= W) return;
-  const int fb = lower_bound_dev<int32_t>(rowid, 0, T, (int32_t)w);
-  const int lb = lower_bound_dev<int32_t>(rowid, fb, T, (int32_t)(w + 1));
+  const int fb = wave_lower_bound<int32_t>(rowid, 0, T, (int32_t)w);
+  const int lb = wave_lower_bound<int32_t>(rowid, fb, T, (int32_t)(w + 1));
   if (fb < lb && lb - fb <= chunk) return;
   const int64_t r0 = w * kBlkH;
   const int64_t r1 = (r0 + kBlkH < (int64_t)M) ? r0 + kBlkH : (int64_t)M;

@@ -186,6 +186,33 @@ dtc_status_t dtc_sddmm_csr(const int32_t *node_pointer, const int32_t *edge_list
 dtc_status_t dtc_gather_f32(const float *in, const int32_t *idx, int32_t count, float *out,
                             dtc_stream_t stream);
 
+/* --------------------------------------------------------- variant selector
+ *
+ * Training and inference kernels of the MLP that picks the SpMM variant per
+ * matrix (the reference's counterpart is the hand-written ExecutionPlan table,
+ * scripts/DTCSpMM/run_DTC_SpMM.py:10-30).  Row-major float32 throughout;
+ * weights are [in_dim x out_dim] and y = act(x W + bias).
+ */
+
+/* y = x W + bias (bias may be NULL); relu != 0 applies max(0, .). */
+dtc_status_t dtc_mlp_linear_forward(const float *x, const float *w, const float *bias, float *y,
+                                    int32_t batch, int32_t in_dim, int32_t out_dim, int32_t relu,
+                                    dtc_stream_t stream);
+/* Gradients of a layer from dy and its output y (y gives the ReLU mask):
+ * dw = x^T dz, db = sum_b dz, dx = dz W^T (dx may be NULL), dz = dy * act'(y). */
+dtc_status_t dtc_mlp_linear_backward(const float *x, const float *w, const float *y,
+                                     const float *dy, float *dx, float *dw, float *db,
+                                     int32_t batch, int32_t in_dim, int32_t out_dim,
+                                     int32_t relu, dtc_stream_t stream);
+/* *loss = inv_count * sum mask (pred - target)^2 (device scalar),
+ * dpred = 2 inv_count mask (pred - target). */
+dtc_status_t dtc_mlp_mse(const float *pred, const float *target, const float *mask, int64_t n,
+                         float inv_count, float *dpred, float *loss, dtc_stream_t stream);
+/* One Adam update of param[n] (bias-corrected, step >= 1). */
+dtc_status_t dtc_adam_step(float *param, const float *grad, float *m, float *v, int64_t n,
+                           float lr, float beta1, float beta2, float eps, int32_t step,
+                           dtc_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

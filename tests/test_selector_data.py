@@ -1,0 +1,64 @@
+"""CPU side of the selector: the float64 MLP oracle is pinned by central
+finite differences, and the committed timing dataset / model are checked
+for shape and consistency with the variant list."""
+import os
+
+import numpy as np
+import pytest
+
+import mlp_oracle as mo
+
+
+def _params(rng, dims):
+    return [(rng.standard_normal((dims[i], dims[i + 1])) * 0.5, rng.standard_normal(dims[i + 1]) * 0.1)
+            for i in range(3)]
+
+
+def test_mlp_oracle_gradients_finite_differences():
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((7, 5))
+    target = rng.standard_normal((7, 4))
+    mask = (rng.random((7, 4)) > 0.2).astype(np.float64)
+    params = _params(rng, [5, 6, 6, 4])
+    loss, grads = mo.mlp_loss_and_grads(x, target, mask, params)
+    h = 1e-6
+    for li in range(3):
+        for pi in range(2):
+            p = params[li][pi]
+            g = grads[li][pi]
+            it = np.nditer(p, flags=["multi_index"])
+            for _ in it:
+                idx = it.multi_index
+                old = p[idx]
+                p[idx] = old + h
+                lp, _ = mo.mlp_loss_and_grads(x, target, mask, params)
+                p[idx] = old - h
+                lm, _ = mo.mlp_loss_and_grads(x, target, mask, params)
+                p[idx] = old
+                assert abs((lp - lm) / (2 * h) - g[idx]) <= 1e-6 * max(1.0, abs(g[idx])), (li, pi, idx)
+
+
+def test_adam_oracle_first_step():
+    # With bias correction the first Adam step moves each weight by lr * sign(g).
+    p = np.array([1.0, -2.0, 3.0])
+    g = np.array([0.5, -0.25, 1e-3])
+    new, m, v = mo.adam(p, g, np.zeros(3), np.zeros(3), 1, lr=0.1)
+    assert np.allclose(new, p - 0.1 * np.sign(g), atol=1e-6)
+
+
+def test_dataset_and_model_consistent():
+    from dtc.selector import DEFAULT_DATA, DEFAULT_MODEL, FEATURE_NAMES, VARIANTS, variant_name
+    if not os.path.exists(DEFAULT_DATA):
+        pytest.skip("no committed selector dataset")
+    z = np.load(DEFAULT_DATA)
+    n = z["features"].shape[0]
+    assert z["features"].shape == (n, len(FEATURE_NAMES))
+    assert z["times_ms"].shape == (n, len(VARIANTS))
+    assert [str(s) for s in z["variants"]] == [variant_name(v) for v in VARIANTS]
+    assert np.isfinite(z["features"]).all()
+    assert (np.isfinite(z["times_ms"]) & (z["times_ms"] > 0)).mean() > 0.99
+    assert len(np.unique(z["seed"])) == n
+    if os.path.exists(DEFAULT_MODEL):
+        m = np.load(DEFAULT_MODEL)
+        assert [str(s) for s in m["variants"]] == [variant_name(v) for v in VARIANTS]
+        assert m["w0"].shape[0] == len(FEATURE_NAMES) and m["w2"].shape[1] == len(VARIANTS)
