@@ -60,6 +60,71 @@ def copy_probe():
     print(json.dumps({"copy_probe_bytes": 1 << 30, "launches": 3}), flush=True)
 
 
+def bench_gcn(args):
+    """GCN layer H = A (X W): R-MAT scale 18 (262,144 nodes, edge factor 16,
+    symmetric normalisation), X 262,144 x 256 and W 256 x 256 (Xavier) in
+    bf16, float32 accumulation.  Times the forward, forward + backward and the
+    SpMM alone, against the same layer with float32 X, W and B."""
+    import numpy as np
+    import torch
+
+    from dtc.gcn import gcn_layer
+    from dtc.graphs import rmat
+    from dtc.matrix import DTCMatrix, spmm_autograd
+    from dtc.metrics import spmm_flops
+
+    scale = 18 if args.scale == 20 else args.scale
+    F = args.n
+    a = rmat(scale, 16, seed=1)
+    M = a.num_rows
+    A = DTCMatrix(a.rowptr, a.col, None, num_rows=M, num_cols=M)
+    A.transpose()
+    stream = torch.cuda.current_stream()
+    rng = np.random.default_rng(0)
+    lim = np.sqrt(6.0 / (2 * F))
+    X32 = torch.from_numpy(rng.standard_normal((M, F)).astype(np.float32)).cuda()
+    W32 = torch.from_numpy(rng.uniform(-lim, lim, (F, F)).astype(np.float32)).cuda()
+    Xb, Wb = X32.to(torch.bfloat16), W32.to(torch.bfloat16)
+    v = torch.from_numpy(a.values).cuda()
+    dH = torch.randn((M, F), device="cuda")
+
+    def leaf(t):
+        return t.detach().clone().requires_grad_(True)
+
+    vb, Xbl, Wbl = leaf(v), leaf(Xb), leaf(Wb)
+    v32, X32l, W32l = leaf(v), leaf(X32), leaf(W32)
+    runs = {
+        "bf16_forward": lambda: gcn_layer(vb, Xbl, Wbl, A),
+        "bf16_forward_backward": lambda: gcn_layer(vb, Xbl, Wbl, A).backward(dH),
+        "fp32_forward": lambda: spmm_autograd(v32, X32l @ W32l, A),
+        "fp32_forward_backward": lambda: spmm_autograd(v32, X32l @ W32l, A).backward(dH),
+    }
+    vals_tc = A.values_tc if A.values_tc is not None else None
+    XWb = (Xb @ Wb).contiguous()
+    XW32 = (X32 @ W32).contiguous()
+    from dtc import load_extension
+    vtc = load_extension().gather(v, A.tc_edge)
+    runs["spmm_bf16_B"] = lambda: A.spmm(XWb, values_tc=vtc)
+    runs["spmm_fp32_B"] = lambda: A.spmm(XW32, values_tc=vtc)
+    del vals_tc
+    out = {}
+    for name, fn in runs.items():
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        out[name] = round(timed(stream, fn, args.steps), 4)
+        log(f"  {name:24s} {out[name]:8.3f} ms")
+    flops = spmm_flops(a.nnz, F)
+    print(json.dumps({
+        "metric": "GCN layer time (ms), A (X W) with bf16 X, W vs float32",
+        "workload": f"R-MAT scale {scale}, edge factor 16, symmetric normalisation; X {M}x{F}, W {F}x{F}",
+        "nnz": a.nnz, "ms": out,
+        "spmm_gflops": {"bf16_B": round(flops / out["spmm_bf16_B"] / 1e6, 1),
+                        "fp32_B": round(flops / out["spmm_fp32_B"] / 1e6, 1)},
+        "speedup_bf16_layer_fwd_bwd": round(out["fp32_forward_backward"] / out["bf16_forward_backward"], 3),
+    }), flush=True)
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -85,6 +150,11 @@ def main():
     ap.add_argument("--traffic-file", default=os.path.join(REPO, "profiles", "traffic_rmat20_n256.json"))
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy (counter calibration)")
+    ap.add_argument("--workload", default="rmat", choices=["rmat", "uniform", "sbm", "gcn"],
+                    help="rmat: power-law SpMM (default); uniform: 4096^2, 16 per row; "
+                         "sbm: 232,965-row block model with hubs; gcn: layer A (X W), bf16 X and W")
+    ap.add_argument("--chunks", default="",
+                    help="extra balanced FP32 chunk sizes for the table, e.g. 16,4096")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -93,12 +163,15 @@ def main():
         print("bench.py: this DTC-SpMM build runs on one GPU only; multi-GPU runs are not part of it "
               "(DESIGN.md describes how the path would shard).", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "gcn":
+        bench_gcn(args)
+        return
 
     import numpy as np
     import torch
 
     from dtc import selector as sel
-    from dtc.graphs import rmat
+    from dtc.graphs import rmat, sbm_hubs, uniform_rows
     from dtc.matrix import DTCMatrix
     from dtc.metrics import compulsory_bytes, gather_model_bytes, spmm_flops
 
@@ -107,9 +180,19 @@ def main():
     stream = torch.cuda.current_stream()
     N = args.n
     t0 = time.time()
-    a = rmat(args.scale, 16, seed=1)
+    if args.workload == "rmat":
+        a = rmat(args.scale, 16, seed=1)
+        wdesc = (f"R-MAT scale {args.scale} (a=.57,b=.19,c=.19), edge factor 16, symmetrised, "
+                 f"1/sqrt(deg_i deg_j) values")
+    elif args.workload == "uniform":
+        a = uniform_rows(4096, 4096, 16, seed=0)
+        wdesc = "uniform 4096x4096, 16 distinct columns per row (PCG64 seed 0), U[-1,1] values"
+    else:
+        a = sbm_hubs()
+        wdesc = ("block model, 232,965 rows, 50 Zipf(1.1) communities, ~490 per row, 5% cross, "
+                 "200 hub rows of 20k-50k, U[0,1] values")
     M, nnz = a.num_rows, a.nnz
-    log(f"R-MAT scale {args.scale}: M={M} nnz={nnz} generated in {time.time() - t0:.1f}s")
+    log(f"{args.workload}: M={M} nnz={nnz} generated in {time.time() - t0:.1f}s")
 
     # ---- ME-TCF on the GPU (timed apart from the SpMM)
     rowptr_d = torch.from_numpy(a.rowptr).to(dev)
@@ -165,6 +248,7 @@ def main():
     table_variants = list(sel.VARIANTS)
     if not args.no_table:
         table_variants += [("balanced", p, 256, 4, 1, False) for p in EXTRA_PRECISIONS]
+        table_variants += [("balanced", "fp32", int(c), 4, 1, False) for c in args.chunks.split(",") if c]
     table = []
     err_cache = {}
     if not args.no_table:
@@ -252,7 +336,7 @@ def main():
             oracle.metcf(a.rowptr, a.col, M, values=a.values, nthreads=cores)
             tb.append(time.perf_counter() - t)
         cpu = {"value": round(flops / t_cpu / 1e9, 2), "unit": "GFLOP/s", "cores": cores, "kind": "port",
-               "sample": f"full SpMM of this workload (R-MAT scale {args.scale}, N={N}) with the CPU "
+               "sample": f"full SpMM of this workload ({args.workload}, N={N}) with the CPU "
                          f"oracle's float32 ME-TCF port, median of 3 runs; host ME-TCF build "
                          f"{np.median(tb) * 1e3:.0f} ms", "cpu_model": _cpu_model()}
 
@@ -270,9 +354,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": dtypes[chosen_v[1]],
-        "data": "synthetic (seeded R-MAT graph, B from torch N(0,1) seed 2)",
-        "config": {"workload": f"R-MAT scale {args.scale} (a=.57,b=.19,c=.19), edge factor 16, "
-                               f"symmetrised, 1/sqrt(deg_i deg_j) values; B {M}x{N} f32",
+        "data": f"synthetic (seeded {args.workload} matrix, B from torch N(0,1) seed 2)",
+        "config": {"workload": f"{wdesc}; B {M}x{N} f32",
                    "num_rows": M, "nnz": nnz, "N": N, "tc_blocks": T, "row_windows": W,
                    "sum_unique_columns": sum_u, "variant": sel.variant_name(chosen_v), "chosen_by": how,
                    "parallelism": "single GPU"},

@@ -96,7 +96,8 @@ void check_operands(const Operands &o, int64_t num_nodes) {
   CHECK_INPUT(o.tile);
   CHECK_INPUT(o.off);
   CHECK_INPUT(o.atox);
-  CHECK_DTYPE(o.input, torch::kFloat32);
+  TORCH_CHECK(o.input.scalar_type() == torch::kFloat32 || o.input.scalar_type() == torch::kBFloat16,
+              "input must be float32 (or bfloat16 for spmm), got ", o.input.scalar_type());
   CHECK_DTYPE(o.tile, torch::kUInt8);
   CHECK_DTYPE(o.off, torch::kInt32);
   CHECK_DTYPE(o.atox, torch::kInt32);
@@ -135,7 +136,7 @@ Tensor output_for(const Tensor &input, int64_t num_nodes, bool ref_shape, const 
   }
   if (!ref_shape) {
     *m_eff = num_nodes;
-    return torch::empty({num_nodes, input.size(1)}, input.options());
+    return torch::empty({num_nodes, input.size(1)}, input.options().dtype(torch::kFloat32));
   }
   const int64_t rows = input.size(0);
   *m_eff = std::min<int64_t>(num_nodes, rows);
@@ -267,7 +268,8 @@ std::vector<Tensor> run_windows(const Operands &o, const Tensor &rwo, int64_t nu
   int64_t m_eff = 0;
   Tensor c = output_for(o.input, num_nodes, ref_shape, out, &m_eff);
   const int64_t n = o.input.size(1);
-  opt.vec = fit_vec(opt.vec, n, o.input, c);
+  if (o.input.scalar_type() == torch::kBFloat16) opt.b_dtype = DTC_B_BFLOAT16;
+  else opt.vec = fit_vec(opt.vec, n, o.input, c);
   // The reference bounds B rows by num_nodes (dense_bound,
   // DTCSpMM_kernel.cu:2687); the general entry takes B's own row count.
   const int64_t b_rows = ref_shape ? std::min<int64_t>(num_nodes, o.input.size(0)) : o.input.size(0);
@@ -276,7 +278,7 @@ std::vector<Tensor> run_windows(const Operands &o, const Tensor &rwo, int64_t nu
     check_status(dtc_spmm_windows(rwo.data_ptr<int32_t>(), o.tile.data_ptr<uint8_t>(), o.off.data_ptr<int32_t>(),
                                   o.atox.data_ptr<int32_t>(), vals, (int32_t)W, (int32_t)(o.off.numel() - 1),
                                   (int32_t)o.tile.numel(), (int32_t)m_eff,
-                                  o.input.data_ptr<float>(), (int32_t)b_rows, (int32_t)n, c.data_ptr<float>(),
+                                  o.input.data_ptr(), (int32_t)b_rows, (int32_t)n, c.data_ptr<float>(),
                                   &opt, current_stream()),
                  "run_DTCSpMM");
   };
@@ -297,7 +299,8 @@ std::vector<Tensor> run_balanced(const Operands &o, const Tensor &rowid, int64_t
   int64_t m_eff = 0;
   Tensor c = output_for(o.input, num_nodes, ref_shape, out, &m_eff);
   const int64_t n = o.input.size(1);
-  opt.vec = fit_vec(opt.vec, n, o.input, c);
+  if (o.input.scalar_type() == torch::kBFloat16) opt.b_dtype = DTC_B_BFLOAT16;
+  else opt.vec = fit_vec(opt.vec, n, o.input, c);
   // The reference bounds B rows by num_nodes (dense_bound,
   // DTCSpMM_kernel.cu:2687); the general entry takes B's own row count.
   const int64_t b_rows = ref_shape ? std::min<int64_t>(num_nodes, o.input.size(0)) : o.input.size(0);
@@ -309,7 +312,7 @@ std::vector<Tensor> run_balanced(const Operands &o, const Tensor &rowid, int64_t
     check_status(dtc_spmm_balanced(rowid.data_ptr<int32_t>(), o.tile.data_ptr<uint8_t>(), o.off.data_ptr<int32_t>(),
                                    o.atox.data_ptr<int32_t>(), vals, (int32_t)T, (int32_t)o.tile.numel(),
                                    (int32_t)m_eff,
-                                   o.input.data_ptr<float>(), (int32_t)b_rows, (int32_t)n, c.data_ptr<float>(), &opt,
+                                   o.input.data_ptr(), (int32_t)b_rows, (int32_t)n, c.data_ptr<float>(), &opt,
                                    ws.data_ptr(), ws_bytes, current_stream()),
                  "run_DTCSpMM_balance");
   };
@@ -323,6 +326,7 @@ std::vector<Tensor> run_balanced(const Operands &o, const Tensor &rowid, int64_t
 std::vector<Tensor> run_DTCSpMM(Tensor input, Tensor Rowwindow_offset, Tensor TCblocktile_id, Tensor TCblock_offset,
                                 Tensor sparse_AToX_idx, int64_t num_nodes, int64_t num_edges, std::string exeplan,
                                 c10::optional<Tensor> values, std::string precision) {
+  CHECK_DTYPE(input, torch::kFloat32);
   dtc_spmm_options_t opt{};
   parse_exeplan(exeplan, opt);
   opt.precision = parse_precision(precision);
@@ -336,6 +340,7 @@ std::vector<Tensor> run_DTCSpMM_balance(Tensor input, Tensor TCblock_rowid, Tens
                                         Tensor TCblock_offset, Tensor sparse_AToX_idx, int64_t num_nodes,
                                         std::string exeplan, c10::optional<Tensor> values, std::string precision,
                                         int64_t chunk_blocks, bool persistent) {
+  CHECK_DTYPE(input, torch::kFloat32);
   dtc_spmm_options_t opt{};
   parse_exeplan(exeplan, opt);
   opt.precision = parse_precision(precision);
@@ -351,6 +356,7 @@ std::vector<Tensor> run_DTCSpMM_balance(Tensor input, Tensor TCblock_rowid, Tens
 std::vector<Tensor> DTCSpMM_gcn(Tensor input, Tensor Rowwindow_offset, Tensor TCblocktile_id, Tensor TCblock_offset,
                                 Tensor sparse_AToX_idx, int64_t num_nodes, int64_t num_edges,
                                 c10::optional<Tensor> values) {
+  CHECK_DTYPE(input, torch::kFloat32);
   dtc_spmm_options_t opt{};
   const int64_t n = input.dim() == 2 ? input.size(1) : 0;
   if (n >= 128 && n % 4 == 0) { opt.vec = 4; opt.subtiles = 1; }
@@ -368,7 +374,7 @@ Tensor spmm(Tensor input, Tensor window_or_rowid, Tensor TCblocktile_id, Tensor 
             int64_t subtiles, int64_t chunk_blocks, bool persistent, c10::optional<Tensor> out) {
   dtc_spmm_options_t opt{};
   opt.precision = parse_precision(precision);
-  TORCH_CHECK_VALUE(vec == 0 || vec == 1 || vec == 2 || vec == 4, "vec must be 0, 1, 2 or 4");
+  TORCH_CHECK_VALUE(vec == 0 || vec == 1 || vec == 2 || vec == 4 || vec == 8, "vec must be 0, 1, 2, 4 or 8");
   TORCH_CHECK_VALUE(subtiles >= 0 && subtiles <= 2, "subtiles must be 0, 1 or 2");
   opt.vec = (int32_t)vec;
   opt.subtiles = (int32_t)subtiles;

@@ -24,6 +24,7 @@
 // products.
 #include <algorithm>
 #include <string>
+#include <type_traits>
 
 #include "dtc_internal.h"
 
@@ -47,7 +48,7 @@ struct SpmmParams {
   const int32_t *off;
   const int32_t *atox;
   const float *vals;  // TC-block order, NULL = all ones
-  const float *B;
+  const void *B;      // float32, or bf16 for the bf16-B schemes
   float *C;
   int32_t num_windows;
   int32_t num_blocks;
@@ -68,7 +69,7 @@ struct SpmmParams {
   const int32_t *__restrict__ a_rwo, const int32_t *__restrict__ a_rowid,                  \
       const uint8_t *__restrict__ a_tile, const int32_t *__restrict__ a_off,               \
       const int32_t *__restrict__ a_atox, const float *__restrict__ a_vals,                \
-      const float *__restrict__ a_B, float *__restrict__ a_C, SpmmParams a_p
+      const void *__restrict__ a_B, float *__restrict__ a_C, SpmmParams a_p
 #define SPMM_PARAMS_FROM_ARGS                                                              \
   SpmmParams {                                                                             \
     a_rwo, a_rowid, a_tile, a_off, a_atox, a_vals, a_B, a_C, a_p.num_windows,              \
@@ -101,6 +102,25 @@ __device__ __forceinline__ typename VecType<VEC>::type vzero() {
   return z;
 }
 
+// Schemes for B stored as bf16 (values taken exactly): A enters as three
+// bf16 limbs (A3) or as one (A1); accumulation is float32 either way.
+constexpr int kBf16B_A3 = 16;
+constexpr int kBf16B_A1 = 17;
+template <int PREC>
+constexpr bool is_bf16b() {
+  return PREC == kBf16B_A3 || PREC == kBf16B_A1;
+}
+
+template <int VEC>
+struct BfVec {
+  typedef __bf16 type __attribute__((ext_vector_type(VEC)));
+};
+
+// Register type of one lane's slice of one gathered B row.
+template <int PREC, int VEC>
+using breg_t = typename std::conditional<is_bf16b<PREC>(), typename BfVec<VEC>::type,
+                                         typename VecType<VEC>::type>::type;
+
 __device__ __forceinline__ float bf16_to_f32(__bf16 x) { return (float)x; }
 
 // Limb split of x into bf16 terms, x = l0 + l1 (+ l2) + residual.
@@ -121,10 +141,33 @@ __device__ __forceinline__ void split3(float x, __bf16 &h, __bf16 &m, __bf16 &l)
 
 // acc[s][v] += A_group * B_group for the columns of subtile s, component v.
 template <int PREC, int VEC, int NSUB>
-__device__ __forceinline__ void mfma_group(const float (&a)[8],
-                                           const typename VecType<VEC>::type (&breg)[NSUB][8],
+__device__ __forceinline__ void mfma_group(const float (&a)[8], const breg_t<PREC, VEC> (&breg)[NSUB][8],
                                            floatx4 (&acc)[NSUB][VEC], bool has_vals) {
-  if constexpr (PREC == DTC_PRECISION_FP32) {
+  if constexpr (is_bf16b<PREC>()) {
+    bf16x8 A0, A1, A2;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      __bf16 h, m, l;
+      split3(a[j], h, m, l);
+      A0[j] = h;
+      A1[j] = m;
+      A2[j] = l;
+    }
+#pragma unroll
+    for (int s = 0; s < NSUB; ++s)
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+        bf16x8 Bv;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) Bv[j] = breg[s][j][v];
+        floatx4 c = acc[s][v];
+        if (PREC == kBf16B_A3 && has_vals) {
+          c = MFMA_BF16(A2, Bv, c);
+          c = MFMA_BF16(A1, Bv, c);
+        }
+        acc[s][v] = MFMA_BF16(A0, Bv, c);
+      }
+  } else if constexpr (PREC == DTC_PRECISION_FP32) {
 #pragma unroll
     for (int j = 0; j < 8; ++j)
 #pragma unroll
@@ -271,11 +314,18 @@ __device__ __forceinline__ void meta_rows(const GroupMeta &m, int (&rows)[8]) {
 
 // B rows of the group into registers: loads from clamped addresses, then
 // zero for slots without a row or columns past N (no branches around loads).
-template <int VEC, int NSUB>
+template <int PREC, int VEC, int NSUB>
 __device__ __forceinline__ void gather_b(const SpmmParams &p, const int (&rows)[8], int colbase,
-                                         typename VecType<VEC>::type (&breg)[NSUB][8]) {
-  typedef typename VecType<VEC>::type vfloat;
+                                         breg_t<PREC, VEC> (&breg)[NSUB][8]) {
+  typedef breg_t<PREC, VEC> vt;
+  typedef typename std::conditional<is_bf16b<PREC>(), __bf16, float>::type et;
+  const et *B = static_cast<const et *>(p.B);
   const int64_t N = p.N;
+  vt zero;
+  if constexpr (VEC == 1 && !is_bf16b<PREC>()) zero = 0.f;
+  else
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) zero[v] = (et)0.f;
 #pragma unroll
   for (int s = 0; s < NSUB; ++s) {
     const int col = colbase + s * 16 * VEC;
@@ -286,8 +336,8 @@ __device__ __forceinline__ void gather_b(const SpmmParams &p, const int (&rows)[
       const int r = rows[j];
       const bool ok = colok && (unsigned)r < (unsigned)p.b_rows;
       const int rc = ok ? r : 0;
-      const vfloat v = *reinterpret_cast<const vfloat *>(p.B + (int64_t)rc * N + colc);
-      breg[s][j] = ok ? v : vzero<VEC>();
+      const vt v = *reinterpret_cast<const vt *>(B + (int64_t)rc * N + colc);
+      breg[s][j] = ok ? v : zero;
     }
   }
 }
@@ -332,15 +382,14 @@ template <int PREC, int VEC, int NSUB>
 __device__ __forceinline__ void run_blocks(const SpmmParams &p, int b0, int b1, int colbase,
                                            float *lds, floatx4 (&acc)[NSUB][VEC],
                                            bool has_vals) {
-  typedef typename VecType<VEC>::type vfloat;
   if (b0 >= b1) return;
   GroupMeta cur;
   load_meta(p, b0, b1, has_vals, cur);
   for (int bb = b0; bb < b1; bb += kGroup) {
-    vfloat breg[NSUB][8];
+    breg_t<PREC, VEC> breg[NSUB][8];
     int rows[8];
     meta_rows(cur, rows);
-    gather_b<VEC, NSUB>(p, rows, colbase, breg);
+    gather_b<PREC, VEC, NSUB>(p, rows, colbase, breg);
     // The last group re-reads its own metadata instead of branching.
     const int bn = (bb + kGroup < b1) ? bb + kGroup : bb;
     GroupMeta nxt;
@@ -519,9 +568,10 @@ __global__ __launch_bounds__(256) void k_balance_prep(const int32_t *rowid, int3
 // ------------------------------------------------------------ host side
 
 typedef void (*KernelFn)(const int32_t *, const int32_t *, const uint8_t *, const int32_t *,
-                         const int32_t *, const float *, const float *, float *, SpmmParams);
+                         const int32_t *, const float *, const void *, float *, SpmmParams);
 
 struct Variant {
+  int prec;  // DTC_PRECISION_* or a bf16-B scheme
   int vec;
   int nsub;
 };
@@ -543,20 +593,48 @@ KernelFn pick_shape(int vec, int nsub, int schedule) {
   return pick_schedule<PREC, 1, 1>(schedule);
 }
 
-KernelFn pick_kernel(int prec, int vec, int nsub, int schedule) {
-  switch (prec) {
-    case DTC_PRECISION_FP32: return pick_shape<DTC_PRECISION_FP32>(vec, nsub, schedule);
-    case DTC_PRECISION_BF16: return pick_shape<DTC_PRECISION_BF16>(vec, nsub, schedule);
-    case DTC_PRECISION_FP16: return pick_shape<DTC_PRECISION_FP16>(vec, nsub, schedule);
-    case DTC_PRECISION_BF16X3: return pick_shape<DTC_PRECISION_BF16X3>(vec, nsub, schedule);
-    default: return pick_shape<DTC_PRECISION_BF16X6>(vec, nsub, schedule);
+template <int PREC>
+KernelFn pick_shape_bf16b(int vec, int schedule) {
+  if (vec == 8) return pick_schedule<PREC, 8, 1>(schedule);
+  return pick_schedule<PREC, 4, 1>(schedule);
+}
+
+KernelFn pick_kernel(const Variant &v, int schedule) {
+  switch (v.prec) {
+    case DTC_PRECISION_FP32: return pick_shape<DTC_PRECISION_FP32>(v.vec, v.nsub, schedule);
+    case DTC_PRECISION_BF16: return pick_shape<DTC_PRECISION_BF16>(v.vec, v.nsub, schedule);
+    case DTC_PRECISION_FP16: return pick_shape<DTC_PRECISION_FP16>(v.vec, v.nsub, schedule);
+    case DTC_PRECISION_BF16X3: return pick_shape<DTC_PRECISION_BF16X3>(v.vec, v.nsub, schedule);
+    case kBf16B_A3: return pick_shape_bf16b<kBf16B_A3>(v.vec, schedule);
+    case kBf16B_A1: return pick_shape_bf16b<kBf16B_A1>(v.vec, schedule);
+    default: return pick_shape<DTC_PRECISION_BF16X6>(v.vec, v.nsub, schedule);
   }
 }
 
-dtc_status_t resolve_shape(const dtc_spmm_options_t *opt, int32_t n, const float *b,
-                           const float *c, Variant *out) {
+// Kernel scheme and lane shape from the options, N and operand alignment.
+dtc_status_t resolve_shape(const dtc_spmm_options_t *opt, int32_t n, const void *b, const float *c,
+                           Variant *out) {
+  const int prec = opt ? opt->precision : DTC_PRECISION_FP32;
+  const int bdt = opt ? opt->b_dtype : DTC_B_FLOAT32;
   int vec = opt ? opt->vec : 0;
   int nsub = opt ? opt->subtiles : 0;
+  if (bdt == DTC_B_BFLOAT16) {
+    // bf16 B: 4 or 8 bf16 per lane (8 or 16 bytes), one subtile.
+    if (vec != 0 && vec != 4 && vec != 8)
+      return fail(DTC_ERROR_INVALID_VALUE, "vec must be 0, 4 or 8 for bfloat16 B");
+    auto fits = [&](int v) {
+      return n % v == 0 && ((uintptr_t)b % (2 * v)) == 0 && ((uintptr_t)c % (4 * v)) == 0;
+    };
+    if (vec == 0) vec = fits(8) ? 8 : 4;
+    if (!fits(vec))
+      return fail(DTC_ERROR_INVALID_VALUE,
+                  "bfloat16 B needs N to be a multiple of 4 and 8-byte aligned B/C rows");
+    out->prec = (prec == DTC_PRECISION_BF16) ? kBf16B_A1 : kBf16B_A3;
+    out->vec = vec;
+    out->nsub = 1;
+    return DTC_SUCCESS;
+  }
+  if (bdt != DTC_B_FLOAT32) return fail(DTC_ERROR_INVALID_VALUE, "unknown b_dtype");
   if (vec != 0 && vec != 1 && vec != 2 && vec != 4)
     return fail(DTC_ERROR_INVALID_VALUE, "vec must be 0, 1, 2 or 4");
   if (nsub < 0 || nsub > 2) return fail(DTC_ERROR_INVALID_VALUE, "subtiles must be 0, 1 or 2");
@@ -569,13 +647,14 @@ dtc_status_t resolve_shape(const dtc_spmm_options_t *opt, int32_t n, const float
                 "vec " + std::to_string(vec) + " needs N and the B/C base addresses to be multiples of it");
   if (nsub == 0) nsub = (vec == 4 && n >= 128) ? 2 : 1;
   if (nsub == 2 && vec != 4) nsub = 1;
+  out->prec = prec;
   out->vec = vec;
   out->nsub = nsub;
   return DTC_SUCCESS;
 }
 
 dtc_status_t check_common(const uint8_t *tile, const int32_t *off, const int32_t *atox,
-                          int32_t num_blocks, int32_t nnz, int32_t num_nodes, const float *b,
+                          int32_t num_blocks, int32_t nnz, int32_t num_nodes, const void *b,
                           int32_t b_rows, int32_t n, float *c, const dtc_spmm_options_t *opt) {
   if (num_nodes < 0 || b_rows < 0 || num_blocks < 0 || nnz < 0)
     return fail(DTC_ERROR_INVALID_VALUE, "sizes must be >= 0");
@@ -624,7 +703,7 @@ dtc_status_t dtc_spmm_windows(const int32_t *row_window_offset, const uint8_t *t
                               const int32_t *tcblock_offset, const int32_t *sparse_atox,
                               const float *tcblock_values, int32_t num_windows,
                               int32_t num_blocks, int32_t nnz,
-                              int32_t num_nodes, const float *b, int32_t b_rows, int32_t n,
+                              int32_t num_nodes, const void *b, int32_t b_rows, int32_t n,
                               float *c, const dtc_spmm_options_t *options,
                               dtc_stream_t stream_) {
   clear_error();
@@ -663,8 +742,7 @@ dtc_status_t dtc_spmm_windows(const int32_t *row_window_offset, const uint8_t *t
   p.num_items = W;
   p.num_blocks = num_blocks;
   p.nnz = nnz;
-  const int prec = options ? options->precision : DTC_PRECISION_FP32;
-  KernelFn fn = pick_kernel(prec, var.vec, var.nsub, 0);
+  KernelFn fn = pick_kernel(var, 0);
   const int64_t grid = (int64_t)p.n_ct * ceil_div(W, kWavesPerBlock);
   if (grid > INT32_MAX) return fail(DTC_ERROR_UNSUPPORTED, "grid too large");
   hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(256), 0, stream, p.rwo, p.rowid, p.tile, p.off,
@@ -684,7 +762,7 @@ dtc_status_t dtc_spmm_balanced_workspace_size(int32_t num_blocks, size_t *bytes)
 dtc_status_t dtc_spmm_balanced(const int32_t *tcblock_rowid, const uint8_t *tcblocktile_id,
                                const int32_t *tcblock_offset, const int32_t *sparse_atox,
                                const float *tcblock_values, int32_t num_blocks, int32_t nnz,
-                               int32_t num_nodes, const float *b, int32_t b_rows, int32_t n,
+                               int32_t num_nodes, const void *b, int32_t b_rows, int32_t n,
                                float *c, const dtc_spmm_options_t *options, void *workspace,
                                size_t workspace_bytes, dtc_stream_t stream_) {
   clear_error();
@@ -726,9 +804,8 @@ dtc_status_t dtc_spmm_balanced(const int32_t *tcblock_rowid, const uint8_t *tcbl
       tcblock_rowid, num_blocks, W, num_nodes, n, chunk, c, p.queue);
   DTC_LAUNCH_CHECK("balance_prep");
   if (num_blocks == 0) return DTC_SUCCESS;
-  const int prec = options ? options->precision : DTC_PRECISION_FP32;
   if (!persistent) {
-    KernelFn fn = pick_kernel(prec, var.vec, var.nsub, 1);
+    KernelFn fn = pick_kernel(var, 1);
     const int64_t grid = (int64_t)p.n_ct * ceil_div(p.num_items, kWavesPerBlock);
     if (grid > INT32_MAX) return fail(DTC_ERROR_UNSUPPORTED, "grid too large");
     hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(256), 0, stream, p.rwo, p.rowid, p.tile, p.off,
@@ -740,7 +817,7 @@ dtc_status_t dtc_spmm_balanced(const int32_t *tcblock_rowid, const uint8_t *tcbl
     const int64_t units = (int64_t)p.num_items * p.n_ct;
     const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(ceil_div(units, kWavesPerBlock),
                                                                 (int64_t)cus * 8));
-    KernelFn fn = pick_kernel(prec, var.vec, var.nsub, 2);
+    KernelFn fn = pick_kernel(var, 2);
     hipLaunchKernelGGL(fn, dim3((unsigned)grid), dim3(256), 0, stream, p.rwo, p.rowid, p.tile, p.off,
                      p.atox, p.vals, p.B, p.C, p);
   }

@@ -1,0 +1,58 @@
+"""GCN-style layer H = A * (X W) over ME-TCF, differentiable in A's values,
+X and W.
+
+X W is a plain dense GEMM (torch / hipBLASLt, bf16 operands, float32
+accumulation, bf16 result); its bf16 output is gathered by the SpMM as is
+(b_dtype bfloat16: half the bytes of a float32 B), with A's values split into
+three bf16 limbs and float32 accumulation.  The backward pass:
+    dXW     = A^T dH            (ME-TCF of A^T, float32 SpMM)
+    dX      = dXW W^T,  dW = X^T dXW   (dense GEMMs)
+    dvalues = SDDMM(dH, XW)     on A's pattern.
+The reference's GCN entry (`DTCSpMM_gcn`, DTCSpMM_kernel.cu:1033-1116) is a
+forward only and its "backward" names return A * dY (DTCSpMM.cpp:384-399).
+"""
+from __future__ import annotations
+
+import torch
+
+from . import load_extension
+from .matrix import DTCMatrix
+
+
+class _GCNFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, values, X, W, A: DTCMatrix, schedule):
+        ext = load_extension()
+        XW = torch.matmul(X, W).contiguous()
+        vals_tc = ext.gather(values.contiguous(), A.tc_edge)
+        H = A.spmm(XW, schedule=schedule, precision="bf16x6", values_tc=vals_tc)
+        ctx.save_for_backward(values, X, W, XW)
+        ctx.A = A
+        ctx.schedule = schedule
+        return H
+
+    @staticmethod
+    def backward(ctx, dH):
+        ext = load_extension()
+        values, X, W, XW = ctx.saved_tensors
+        A: DTCMatrix = ctx.A
+        dH = dH.contiguous().float()
+        d_values = d_X = d_W = None
+        if ctx.needs_input_grad[0]:
+            d_values = ext.sddmm_csr(A.rowptr, A.col, dH, XW.float().contiguous())
+        if ctx.needs_input_grad[1] or ctx.needs_input_grad[2]:
+            At = A.transpose()
+            vals_t_tc = ext.gather(ext.gather(values.contiguous(), A.perm_t), At.tc_edge)
+            dXW = At.spmm(dH, schedule=ctx.schedule, precision="fp32", values_tc=vals_t_tc)
+            dXW = dXW.to(X.dtype)
+            if ctx.needs_input_grad[1]:
+                d_X = torch.matmul(dXW, W.t().contiguous())
+            if ctx.needs_input_grad[2]:
+                d_W = torch.matmul(X.t().contiguous(), dXW)
+        return d_values, d_X, d_W, None, None
+
+
+def gcn_layer(values: torch.Tensor, X: torch.Tensor, W: torch.Tensor, A: DTCMatrix,
+              schedule: str = "balanced") -> torch.Tensor:
+    """H = A(values) * (X W); X, W bf16 (or float32), H float32."""
+    return _GCNFunction.apply(values, X, W, A, schedule)

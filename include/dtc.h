@@ -43,12 +43,20 @@ typedef enum dtc_precision {
   DTC_PRECISION_BF16X6 = 4  /* 3-limb bf16 split: all limb products of order <= 2 */
 } dtc_precision_t;
 
+typedef enum dtc_b_dtype {
+  DTC_B_FLOAT32 = 0, /* B is float32 */
+  DTC_B_BFLOAT16 = 1 /* B is bfloat16, taken exactly; A enters as three bf16 limbs
+                        (one with DTC_PRECISION_BF16); float32 accumulation and C */
+} dtc_b_dtype_t;
+
 typedef struct dtc_spmm_options {
   int32_t precision;    /* dtc_precision_t */
-  int32_t vec;          /* B columns per lane per gathered row: 1, 2, 4 (0 = widest N allows) */
-  int32_t subtiles;     /* 1 or 2 column tiles of 16*vec per wave (0 = auto) */
+  int32_t vec;          /* B elements per lane per gathered row: 1, 2, 4 (float32 B);
+                           4, 8 (bfloat16 B); 0 = widest N and alignment allow */
+  int32_t subtiles;     /* 1 or 2 column tiles of 16*vec per wave (0 = auto; float32 B) */
   int32_t chunk_blocks; /* balanced schedule: TC blocks per work chunk (0 = 64) */
   int32_t persistent;   /* balanced schedule: 1 = persistent waves pulling chunks from a queue */
+  int32_t b_dtype;      /* dtc_b_dtype_t */
 } dtc_spmm_options_t;
 
 /* ------------------------------------------------------------------ misc */
@@ -116,8 +124,9 @@ dtc_status_t dtc_metcf_build_blocks(const int32_t *node_pointer, const int32_t *
 
 /* ------------------------------------------------------------ SpMM forward
  *
- * C[M x N] = A * B with A in ME-TCF at 16 x 8 blocks, B [b_rows x N] float32
- * row-major, C [num_nodes x N] float32 row-major.  tcblock_values (TC-block
+ * C[M x N] = A * B with A in ME-TCF at 16 x 8 blocks, B [b_rows x N] row-major
+ * (float32, or bfloat16 with options->b_dtype), C [num_nodes x N] float32
+ * row-major.  tcblock_values (TC-block
  * order, [nnz]) may be NULL: every stored element of A is then 1, as the
  * reference's launchers pass (DTCSpMM_kernel.cu:837, :1140).  Columns of A
  * that are >= b_rows read as zero (the dense_bound guard of
@@ -138,7 +147,7 @@ dtc_status_t dtc_spmm_windows(const int32_t *row_window_offset, const uint8_t *t
                               const int32_t *tcblock_offset, const int32_t *sparse_atox,
                               const float *tcblock_values, int32_t num_windows,
                               int32_t num_blocks, int32_t nnz,
-                              int32_t num_nodes, const float *b, int32_t b_rows, int32_t n,
+                              int32_t num_nodes, const void *b, int32_t b_rows, int32_t n,
                               float *c, const dtc_spmm_options_t *options,
                               dtc_stream_t stream);
 
@@ -154,7 +163,7 @@ dtc_status_t dtc_spmm_balanced_workspace_size(int32_t num_blocks, size_t *bytes)
 dtc_status_t dtc_spmm_balanced(const int32_t *tcblock_rowid, const uint8_t *tcblocktile_id,
                                const int32_t *tcblock_offset, const int32_t *sparse_atox,
                                const float *tcblock_values, int32_t num_blocks, int32_t nnz,
-                               int32_t num_nodes, const float *b, int32_t b_rows, int32_t n,
+                               int32_t num_nodes, const void *b, int32_t b_rows, int32_t n,
                                float *c, const dtc_spmm_options_t *options, void *workspace,
                                size_t workspace_bytes, dtc_stream_t stream);
 
