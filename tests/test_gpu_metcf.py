@@ -136,3 +136,18 @@ def test_wrong_sizes_raise(ext):
     with pytest.raises(RuntimeError, match="dtype"):
         ext.preprocess_gpu(col.long(), rpt, 2, 16, 8, torch.zeros(1, dtype=i32, device=dev),
                            torch.zeros(4, dtype=i32, device=dev), torch.zeros(4, dtype=i32, device=dev))
+
+
+def test_metcf_cache_roundtrip(ext, tmp_path):
+    from dtc.graphs import rmat
+    from dtc.io import load_metcf, save_metcf
+    from dtc.matrix import DTCMatrix
+    a = rmat(10, 8, seed=3)
+    say("metcf cache")
+    A = DTCMatrix(a.rowptr, a.col, a.values, num_rows=a.num_rows, num_cols=a.num_cols)
+    p = str(tmp_path / "a.pt")
+    save_metcf(p, A)
+    A2 = load_metcf(p)
+    B = torch.randn((a.num_rows, 32), device="cuda")
+    assert torch.equal(A.spmm(B, schedule="windows"), A2.spmm(B, schedule="windows"))
+    assert A2.block_count == A.block_count and torch.equal(A2.values_tc, A.values_tc)
