@@ -22,7 +22,8 @@ REPO = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 INCLUDE = os.path.join(REPO, "include")
-HIP_SOURCES = ["primitives.hip", "metcf.hip", "spmm.hip", "backward.hip", "selector.hip"]
+HIP_SOURCES = ["primitives.hip", "metcf.hip", "spmm.hip", "backward.hip", "selector.hip",
+               "reorder.hip"]
 HEADERS = [os.path.join(CSRC, "dtc_internal.h"), os.path.join(INCLUDE, "dtc.h")]
 LIB = os.path.join(HERE, "libdtc.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")

@@ -522,9 +522,155 @@ void adam_step(Tensor param, Tensor grad, Tensor m, Tensor v, int64_t step, doub
                "adam_step");
 }
 
+// ------------------------------------------------------------ row reordering
+
+void check_csr_pair(const Tensor &ptr, const Tensor &idx) {
+  CHECK_INPUT(ptr);
+  CHECK_INPUT(idx);
+  CHECK_DTYPE(ptr, torch::kInt32);
+  CHECK_DTYPE(idx, torch::kInt32);
+  CHECK_SAME_DEVICE(idx, ptr);
+  TORCH_CHECK(ptr.dim() == 1 && ptr.numel() >= 1, "row pointer must be 1-D with at least one entry");
+  // The stage entry points trust their CSR; check it here (one host sync).
+  TORCH_CHECK(ptr[0].item<int32_t>() == 0 && ptr[-1].item<int64_t>() == idx.numel(),
+              "row pointer must start at 0 and end at the number of entries");
+  if (ptr.numel() > 1)
+    TORCH_CHECK((ptr.slice(0, 1) >= ptr.slice(0, 0, -1)).all().item<bool>(), "row pointer must not decrease");
+  if (idx.numel() > 0) TORCH_CHECK(idx.min().item<int32_t>() >= 0, "column indices must be >= 0");
+}
+
+// uint32 signatures held in an int32 tensor [num_rows, num_hashes].
+Tensor minhash(Tensor nodePointer, Tensor edgeList, int64_t num_hashes, int64_t seed) {
+  check_csr_pair(nodePointer, edgeList);
+  const int64_t M = nodePointer.numel() - 1;
+  const c10::OptionalDeviceGuard guard(nodePointer.device());
+  Tensor sig = torch::empty({M, num_hashes}, nodePointer.options());
+  check_status(dtc_minhash_signatures(nodePointer.data_ptr<int32_t>(), edgeList.data_ptr<int32_t>(), (int32_t)M,
+                                      (int32_t)num_hashes, (uint32_t)seed,
+                                      reinterpret_cast<uint32_t *>(sig.data_ptr<int32_t>()), current_stream()),
+               "minhash");
+  return sig;
+}
+
+int64_t default_max_pairs(int64_t n, int64_t bands, int64_t neighbors) {
+  return std::max<int64_t>(1, std::min<int64_t>(n * bands * neighbors, int64_t(1) << 28));
+}
+
+// Packed candidate pairs (u << 32 | v) as int64.
+Tensor lsh_pairs(Tensor signatures, Tensor set_ptr, int64_t bands, int64_t rows_per_band, int64_t max_neighbors,
+                 int64_t max_pairs) {
+  CHECK_INPUT(signatures);
+  CHECK_INPUT(set_ptr);
+  CHECK_DTYPE(signatures, torch::kInt32);
+  CHECK_DTYPE(set_ptr, torch::kInt32);
+  CHECK_SAME_DEVICE(set_ptr, signatures);
+  TORCH_CHECK(signatures.dim() == 2 && set_ptr.numel() == signatures.size(0) + 1,
+              "signatures must be [n, num_hashes] and set_ptr [n + 1]");
+  const int64_t n = signatures.size(0);
+  if (max_pairs <= 0) max_pairs = default_max_pairs(n, bands, max_neighbors);
+  const c10::OptionalDeviceGuard guard(signatures.device());
+  size_t ws_bytes = 0;
+  check_status(dtc_lsh_pairs_workspace_size((int32_t)n, max_pairs, &ws_bytes), "lsh_pairs");
+  Tensor ws = torch::empty({(int64_t)ws_bytes}, signatures.options().dtype(torch::kUInt8));
+  Tensor pairs = torch::empty({max_pairs}, signatures.options().dtype(torch::kInt64));
+  int64_t count = 0;
+  check_status(dtc_lsh_pairs(reinterpret_cast<const uint32_t *>(signatures.data_ptr<int32_t>()),
+                             set_ptr.data_ptr<int32_t>(), (int32_t)n, (int32_t)signatures.size(1), (int32_t)bands,
+                             (int32_t)rows_per_band, (int32_t)max_neighbors,
+                             reinterpret_cast<uint64_t *>(pairs.data_ptr<int64_t>()), max_pairs, &count,
+                             ws.data_ptr(), ws_bytes, current_stream()),
+               "lsh_pairs");
+  return pairs.slice(0, 0, count).clone();
+}
+
+Tensor jaccard_pairs(Tensor set_ptr, Tensor set_idx, Tensor pairs) {
+  check_csr_pair(set_ptr, set_idx);
+  CHECK_INPUT(pairs);
+  CHECK_DTYPE(pairs, torch::kInt64);
+  CHECK_SAME_DEVICE(pairs, set_ptr);
+  const int64_t n = set_ptr.numel() - 1;
+  if (pairs.numel() > 0) {
+    const Tensor u = torch::bitwise_right_shift(pairs, 32), v = torch::bitwise_and(pairs, 0xFFFFFFFFLL);
+    TORCH_CHECK(u.min().item<int64_t>() >= 0 && torch::maximum(u, v).max().item<int64_t>() < n,
+                "pairs name sets out of range");
+  }
+  const c10::OptionalDeviceGuard guard(pairs.device());
+  Tensor sims = torch::empty({pairs.numel()}, set_ptr.options().dtype(torch::kFloat32));
+  check_status(dtc_jaccard_pairs(set_ptr.data_ptr<int32_t>(), set_idx.data_ptr<int32_t>(),
+                                 reinterpret_cast<const uint64_t *>(pairs.data_ptr<int64_t>()), pairs.numel(),
+                                 sims.data_ptr<float>(), current_stream()),
+               "jaccard_pairs");
+  return sims;
+}
+
+// Host tensors in, host roots out.
+Tensor tca_greedy(Tensor pairs, Tensor similarity, Tensor list_ptr, Tensor list_idx, int64_t thres) {
+  for (const Tensor *t : {&pairs, &similarity, &list_ptr, &list_idx}) {
+    TORCH_CHECK(t->device().is_cpu(), "tca_greedy takes host tensors");
+    CHECK_CONTIGUOUS(*t);
+  }
+  CHECK_DTYPE(pairs, torch::kInt64);
+  CHECK_DTYPE(similarity, torch::kFloat32);
+  CHECK_DTYPE(list_ptr, torch::kInt32);
+  CHECK_DTYPE(list_idx, torch::kInt32);
+  TORCH_CHECK(pairs.numel() == similarity.numel(), "one similarity per pair");
+  const int64_t n = list_ptr.numel() - 1;
+  TORCH_CHECK(n >= 0, "list_ptr must have at least one entry");
+  Tensor root = torch::empty({n}, list_ptr.options());
+  check_status(dtc_tca_greedy((int32_t)n, reinterpret_cast<const uint64_t *>(pairs.data_ptr<int64_t>()),
+                              similarity.data_ptr<float>(), pairs.numel(), list_ptr.data_ptr<int32_t>(),
+                              list_idx.data_ptr<int32_t>(), (int32_t)thres, root.data_ptr<int32_t>()),
+               "tca_greedy");
+  return root;
+}
+
+py::tuple tca_reorder(Tensor nodePointer, Tensor edgeList, int64_t thres, int64_t cluster_thres, int64_t bands,
+                      int64_t rows_per_band, int64_t max_neighbors, int64_t seed, int64_t max_pairs) {
+  check_csr_pair(nodePointer, edgeList);
+  const int64_t M = nodePointer.numel() - 1;
+  if (max_pairs <= 0) max_pairs = default_max_pairs(M, bands, max_neighbors);
+  dtc_tca_options_t o{};
+  o.thres = (int32_t)thres;
+  o.cluster_thres = (int32_t)cluster_thres;
+  o.bands = (int32_t)bands;
+  o.rows_per_band = (int32_t)rows_per_band;
+  o.max_neighbors = (int32_t)max_neighbors;
+  o.seed = (uint32_t)seed;
+  o.max_pairs = max_pairs;
+  const c10::OptionalDeviceGuard guard(nodePointer.device());
+  size_t ws_bytes = 0;
+  check_status(dtc_tca_reorder_workspace_size((int32_t)M, (int32_t)edgeList.numel(), &o, &ws_bytes), "tca_reorder");
+  Tensor ws = torch::empty({(int64_t)ws_bytes}, nodePointer.options().dtype(torch::kUInt8));
+  Tensor order = torch::empty({M}, torch::TensorOptions().dtype(torch::kInt32));
+  dtc_tca_stats_t stats{};
+  check_status(dtc_tca_reorder(nodePointer.data_ptr<int32_t>(), edgeList.data_ptr<int32_t>(), (int32_t)M,
+                               (int32_t)edgeList.numel(), &o, order.data_ptr<int32_t>(), &stats, ws.data_ptr(),
+                               ws_bytes, current_stream()),
+               "tca_reorder");
+  py::dict d;
+  d["row_pairs"] = stats.row_pairs;
+  d["cluster_pairs"] = stats.cluster_pairs;
+  d["clusters"] = stats.clusters;
+  d["super_clusters"] = stats.super_clusters;
+  return py::make_tuple(order, d);
+}
+
 }  // namespace
 
 PYBIND11_MODULE(DTCSpMM, m) {
+  m.def("minhash", &minhash, "MinHash signatures of CSR rows (uint32 bits in int32)", py::arg("nodePointer"),
+        py::arg("edgeList"), py::arg("num_hashes") = 56, py::arg("seed") = 2022);
+  m.def("lsh_pairs", &lsh_pairs, "LSH candidate pairs, packed u << 32 | v", py::arg("signatures"),
+        py::arg("set_ptr"), py::arg("bands") = 28, py::arg("rows_per_band") = 2, py::arg("max_neighbors") = 8,
+        py::arg("max_pairs") = 0);
+  m.def("jaccard_pairs", &jaccard_pairs, "exact Jaccard similarity of set pairs", py::arg("set_ptr"),
+        py::arg("set_idx"), py::arg("pairs"));
+  m.def("tca_greedy", &tca_greedy, "greedy pair-merge clustering on the host -> roots", py::arg("pairs"),
+        py::arg("similarity"), py::arg("list_ptr"), py::arg("list_idx"), py::arg("thres"));
+  m.def("tca_reorder", &tca_reorder, "two-level TCA row reordering -> (reorder_id, stats)", py::arg("nodePointer"),
+        py::arg("edgeList"), py::arg("thres") = 16, py::arg("cluster_thres") = 128, py::arg("bands") = 28,
+        py::arg("rows_per_band") = 2, py::arg("max_neighbors") = 8, py::arg("seed") = 2022,
+        py::arg("max_pairs") = 0);
   m.def("mlp_linear_forward", &mlp_linear_forward, "y = act(x W + b)", py::arg("x"), py::arg("w"),
         py::arg("bias") = py::none(), py::arg("relu") = false);
   m.def("mlp_linear_backward", &mlp_linear_backward, "(dx, dw, db) of one layer", py::arg("x"), py::arg("w"),

@@ -222,6 +222,70 @@ dtc_status_t dtc_adam_step(float *param, const float *grad, float *m, float *v, 
                            float lr, float beta1, float beta2, float eps, int32_t step,
                            dtc_stream_t stream);
 
+/* ------------------------------------------------------------ row reordering
+ *
+ * Replaces the offline TCA reordering script, reordering/TCA_reorder.py
+ * (MinHash/LSH candidates :55-80, Jaccard :122-146, greedy clustering
+ * :152-187, cluster-of-clusters :203-286, output order :300-315).  Rows (and,
+ * at the second level, clusters) are sets of column indices given as CSR with
+ * strictly increasing columns per row.  Signature entry j of a set is
+ * min over its columns c of fmix32(c ^ s_j), s_j = fmix32(j * 0x9E3779B9 +
+ * seed) (fmix32: MurmurHash3's 32-bit finaliser).
+ */
+typedef struct dtc_tca_options {
+  int32_t thres;         /* rows per cluster at which it stops growing (16) */
+  int32_t cluster_thres; /* clusters per super-cluster likewise (128) */
+  int32_t bands;         /* LSH bands (28 for Jaccard threshold 0.2 at 128 perms) */
+  int32_t rows_per_band; /* signature entries per band (2) */
+  int32_t max_neighbors; /* pair rows at most this far apart in a band's order */
+  uint32_t seed;         /* hash family seed */
+  int64_t max_pairs;     /* candidate-pair capacity before deduplication */
+} dtc_tca_options_t;
+
+typedef struct dtc_tca_stats {
+  int64_t row_pairs;     /* unique candidate row pairs */
+  int64_t cluster_pairs; /* unique candidate cluster pairs */
+  int32_t clusters;
+  int32_t super_clusters;
+} dtc_tca_stats_t;
+
+/* signatures[r * num_hashes + j], uint32; empty rows get 0xFFFFFFFF
+ * (replaces MinHash / libMHCUDA.minhash_cuda_calc, TCA_reorder.py:55-80). */
+dtc_status_t dtc_minhash_signatures(const int32_t *node_pointer, const int32_t *edge_list,
+                                    int32_t num_rows, int32_t num_hashes, uint32_t seed,
+                                    uint32_t *signatures, dtc_stream_t stream);
+/* Candidate pairs (u < v, packed u << 32 | v, sorted, unique): some band of
+ * rows_per_band entries agrees and u, v lie at most max_neighbors apart in
+ * that band's stable order; empty sets (set_ptr) never pair (replaces
+ * MinHashLSH.insert/query, TCA_reorder.py:57, :125). */
+dtc_status_t dtc_lsh_pairs_workspace_size(int32_t num_sets, int64_t max_pairs, size_t *bytes);
+dtc_status_t dtc_lsh_pairs(const uint32_t *signatures, const int32_t *set_ptr, int32_t num_sets,
+                           int32_t num_hashes, int32_t bands, int32_t rows_per_band,
+                           int32_t max_neighbors, uint64_t *pairs, int64_t max_pairs,
+                           int64_t *num_pairs, void *workspace, size_t workspace_bytes,
+                           dtc_stream_t stream);
+/* similarity[p] = |S_u ∩ S_v| / |S_u ∪ S_v| in float32 (replaces
+ * cugraph.jaccard, TCA_reorder.py:136). */
+dtc_status_t dtc_jaccard_pairs(const int32_t *set_ptr, const int32_t *set_idx,
+                               const uint64_t *pairs, int64_t num_pairs, float *similarity,
+                               dtc_stream_t stream);
+/* Host memory: the greedy merge of TCA_reorder.py:152-187 over num_items
+ * items (pairs pushed in the given order, most similar popped first, stale
+ * pairs re-queued with the Jaccard of the two roots' own lists); root[i] is
+ * item i's cluster root. */
+dtc_status_t dtc_tca_greedy(int32_t num_items, const uint64_t *pairs, const float *similarity,
+                            int64_t num_pairs, const int32_t *list_ptr, const int32_t *list_idx,
+                            int32_t thres, int32_t *root);
+/* The whole two-level reordering of a device CSR; reorder_id (host,
+ * int32[num_rows]) lists the original row ids in their new order, the content
+ * of the reference's <dataset>.reorder_id.npz (TCA_reorder.py:317). */
+dtc_status_t dtc_tca_reorder_workspace_size(int32_t num_rows, int32_t nnz,
+                                            const dtc_tca_options_t *options, size_t *bytes);
+dtc_status_t dtc_tca_reorder(const int32_t *node_pointer, const int32_t *edge_list,
+                             int32_t num_rows, int32_t nnz, const dtc_tca_options_t *options,
+                             int32_t *reorder_id, dtc_tca_stats_t *stats, void *workspace,
+                             size_t workspace_bytes, dtc_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
