@@ -153,6 +153,8 @@ def main():
     ap.add_argument("--workload", default="rmat", choices=["rmat", "uniform", "sbm", "gcn"],
                     help="rmat: power-law SpMM (default); uniform: 4096^2, 16 per row; "
                          "sbm: 232,965-row block model with hubs; gcn: layer A (X W), bf16 X and W")
+    ap.add_argument("--reorder", action="store_true",
+                    help="also time the chosen variant on the TCA-reordered graph")
     ap.add_argument("--chunks", default="",
                     help="extra balanced FP32 chunk sizes for the table, e.g. 16,4096")
     args = ap.parse_args()
@@ -335,10 +337,56 @@ def main():
             t = time.perf_counter()
             oracle.metcf(a.rowptr, a.col, M, values=a.values, nthreads=cores)
             tb.append(time.perf_counter() - t)
+        # torch.sparse CSR product on the same cores, one run after a warm-up
+        torch.set_num_threads(cores)
+        As = torch.sparse_csr_tensor(torch.from_numpy(a.rowptr.astype(np.int64)),
+                                     torch.from_numpy(a.col.astype(np.int64)),
+                                     torch.from_numpy(a.values), size=(M, M))
+        Bt = torch.from_numpy(Bn)
+        torch.sparse.mm(As, Bt)
+        t = time.perf_counter()
+        torch.sparse.mm(As, Bt)
+        t_ts = time.perf_counter() - t
         cpu = {"value": round(flops / t_cpu / 1e9, 2), "unit": "GFLOP/s", "cores": cores, "kind": "port",
                "sample": f"full SpMM of this workload ({args.workload}, N={N}) with the CPU "
                          f"oracle's float32 ME-TCF port, median of 3 runs; host ME-TCF build "
-                         f"{np.median(tb) * 1e3:.0f} ms", "cpu_model": _cpu_model()}
+                         f"{np.median(tb) * 1e3:.0f} ms", "cpu_model": _cpu_model(),
+               "host_metcf_build_ms": round(float(np.median(tb)) * 1e3, 1),
+               "torch_sparse_csr_gflops": round(flops / t_ts / 1e9, 2)}
+
+    # ---- the same SpMM on the TCA-reordered graph (rows and columns relabelled)
+    reordered = None
+    if args.reorder:
+        from dtc.reorder import relabel, tca_reorder
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        order, rstats = tca_reorder(a.rowptr, a.col, M)
+        t_reorder = time.perf_counter() - t
+        b = relabel(a, order)
+        Ar = DTCMatrix(torch.from_numpy(b.rowptr).to(dev), torch.from_numpy(b.col).to(dev),
+                       torch.from_numpy(b.values).to(dev), num_rows=M, num_cols=M)
+        Br = torch.empty_like(B)
+        Br[torch.from_numpy(np.argsort(order)).to(dev)] = B  # row i of B follows node i
+        Cr = torch.empty_like(C)
+        v = chosen_v
+
+        def fr():
+            return Ar.spmm(Br, schedule=v[0], precision=v[1], chunk_blocks=v[2], vec=v[3], subtiles=v[4],
+                           persistent=v[5], out=Cr)
+        for _ in range(3):
+            fr()
+        r_ms = timed(stream, fr, args.steps)
+        # C of the relabelled graph is C of the original, rows permuted
+        Crn = Cr.cpu().numpy()[np.argsort(order)].astype(np.float64)
+        dr = np.abs(Crn - C64)
+        r_sum_u = Ar.sum_unique_columns()
+        reordered = {"reorder_s": round(t_reorder, 2), "stats": rstats, "sum_unique_columns": r_sum_u,
+                     "tc_blocks": Ar.block_count, "ms": round(r_ms, 4),
+                     "gflops": round(flops / r_ms / 1e6, 2), "speedup": round(ms / r_ms, 3),
+                     "gather_model_frac": round(gather_model_bytes(r_sum_u, M, N, nnz, Ar.block_count, W)
+                                                / (r_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "err": float((dr[mask] / Cabs[mask]).max())}
+        log(f"reordered: sumU {sum_u} -> {r_sum_u}, {ms:.3f} -> {r_ms:.3f} ms, reorder {t_reorder:.1f}s")
 
     dtypes = {"fp32": "f32", "bf16x6": "bf16x6 (f32 accumulate)", "bf16x3": "bf16x3 (f32 accumulate)",
               "fp16": "f16 (f32 accumulate)", "bf16": "bf16 (f32 accumulate)"}
@@ -365,6 +413,7 @@ def main():
         "error_vs_fp64": {"componentwise": comp_err, "normwise": norm_err, "target": ERR_TARGET},
         "fp32_path": fp32,
         "selector": selector_info,
+        "reordered": reordered,
         "preprocess_ms": round(float(np.median(pre_ms)), 3),
         "bytes_model_per_launch": bytes_model,
         "compulsory_bytes": compulsory_bytes(M, M, N, nnz),

@@ -47,6 +47,17 @@ def load_reorder_id(path: str) -> np.ndarray:
     return np.asarray(np.load(path)["reorder_id"], np.int64)
 
 
+def save_reorder_id(path: str, reorder_id) -> None:
+    np.savez(path, reorder_id=np.asarray(reorder_id, np.int64))
+
+
+def save_reordered_graph(path: str, src, dst, num_nodes: int, reorder_id) -> None:
+    """``<dataset>.reorder.npz``: the edge list with both ends relabelled
+    (TCA_reorder.py:318-326)."""
+    s, d = apply_reorder(src, dst, reorder_id)
+    save_graph_npz(path, s, d, num_nodes)
+
+
 def apply_reorder(src, dst, reorder_id):
     """Relabel edge endpoints: node reorder_id[i] becomes node i."""
     reorder_id = np.asarray(reorder_id, np.int64)
