@@ -63,8 +63,9 @@ def copy_probe():
 def bench_gcn(args):
     """GCN layer H = A (X W): R-MAT scale 18 (262,144 nodes, edge factor 16,
     symmetric normalisation), X 262,144 x 256 and W 256 x 256 (Xavier) in
-    bf16, float32 accumulation.  Times the forward, forward + backward and the
-    SpMM alone, against the same layer with float32 X, W and B."""
+    bf16, float32 accumulation.  Times one layer's forward and forward +
+    backward, a 2-layer model A (relu(A (X W1)) W2) forward + backward, and the
+    SpMM alone, against the same computations with float32 X, W and B."""
     import numpy as np
     import torch
 
@@ -93,11 +94,24 @@ def bench_gcn(args):
 
     vb, Xbl, Wbl = leaf(v), leaf(Xb), leaf(Wb)
     v32, X32l, W32l = leaf(v), leaf(X32), leaf(W32)
+    W2_32 = torch.from_numpy(rng.uniform(-lim, lim, (F, F)).astype(np.float32)).cuda()
+    W2bl, W2_32l = leaf(W2_32.to(torch.bfloat16)), leaf(W2_32)
+
+    def two_layer_bf16():
+        H1 = torch.relu(gcn_layer(vb, Xbl, Wbl, A)).to(torch.bfloat16)
+        return gcn_layer(vb, H1, W2bl, A)
+
+    def two_layer_fp32():
+        H1 = torch.relu(spmm_autograd(v32, X32l @ W32l, A))
+        return spmm_autograd(v32, H1 @ W2_32l, A)
+
     runs = {
         "bf16_forward": lambda: gcn_layer(vb, Xbl, Wbl, A),
         "bf16_forward_backward": lambda: gcn_layer(vb, Xbl, Wbl, A).backward(dH),
         "fp32_forward": lambda: spmm_autograd(v32, X32l @ W32l, A),
         "fp32_forward_backward": lambda: spmm_autograd(v32, X32l @ W32l, A).backward(dH),
+        "bf16_2layer_forward_backward": lambda: two_layer_bf16().backward(dH),
+        "fp32_2layer_forward_backward": lambda: two_layer_fp32().backward(dH),
     }
     vals_tc = A.values_tc if A.values_tc is not None else None
     XWb = (Xb @ Wb).contiguous()
@@ -122,6 +136,8 @@ def bench_gcn(args):
         "spmm_gflops": {"bf16_B": round(flops / out["spmm_bf16_B"] / 1e6, 1),
                         "fp32_B": round(flops / out["spmm_fp32_B"] / 1e6, 1)},
         "speedup_bf16_layer_fwd_bwd": round(out["fp32_forward_backward"] / out["bf16_forward_backward"], 3),
+        "speedup_bf16_2layer_fwd_bwd": round(out["fp32_2layer_forward_backward"]
+                                             / out["bf16_2layer_forward_backward"], 3),
     }), flush=True)
 
 
@@ -241,10 +257,13 @@ def main():
     # ---- the learned selector's pick
     feats = sel.features(A, N)
     pick = None
+    ood = []
     if os.path.exists(sel.DEFAULT_MODEL):
         model = sel.MLPSelector.load(sel.DEFAULT_MODEL)
         pick = int(model.choose(feats[None, :])[0])
-        log(f"selector pick: {sel.variant_name(sel.VARIANTS[pick])}")
+        ood = model.out_of_range(feats)
+        log(f"selector pick: {sel.variant_name(sel.VARIANTS[pick])}"
+            + (f" (outside its training range: {', '.join(ood)})" if ood else ""))
 
     # ---- exhaustive table (the selector's judge and fallback)
     table_variants = list(sel.VARIANTS)
@@ -266,13 +285,19 @@ def main():
                               err_norm=err_cache[v[1]][1]))
             log(f"  {sel.variant_name(v):42s} {ms:8.3f} ms  err={err_cache[v[1]][0]:.2e}")
 
-    if args.variant == "selector" and pick is not None:
+    if args.variant == "selector" and pick is not None and not ood:
         chosen_v = sel.VARIANTS[pick]
         how = "selector"
     elif args.variant in ("selector", "exhaustive"):
-        ok = [(r["ms"], i) for i, r in enumerate(table) if r["err"] <= ERR_TARGET]
-        chosen_v = table_variants[min(ok)[1]]
-        how = "exhaustive"
+        # exhaustive timing: the selector's fallback (no model, or a matrix
+        # outside its training range) and the explicit choice
+        if table:
+            ok = [(r["ms"], i) for i, r in enumerate(table) if r["err"] <= ERR_TARGET]
+            chosen_v = table_variants[min(ok)[1]]
+        else:
+            chosen_v = sel.VARIANTS[int(np.nanargmin(sel.time_variants(A, B)))]
+        how = "exhaustive" + (f" (selector fallback: {', '.join(ood)} outside its training range)"
+                              if ood and args.variant == "selector" else "")
     else:
         s_, p_, c_, vec_, sub_ = args.variant.split(":")
         chosen_v = (s_, p_, int(c_), int(vec_), int(sub_), False)
@@ -292,9 +317,11 @@ def main():
     bytes_model = gather_model_bytes(sum_u, M, N, nnz, T, W)
     achieved = bytes_model / (ms * 1e-3) / 1e9
     traffic = None
-    if os.path.exists(args.traffic_file):
+    if args.workload == "rmat" and args.scale == 20 and N == 256 and os.path.exists(args.traffic_file):
         with open(args.traffic_file) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            tj = json.load(f)
+        if tj.get("variant") == sel.variant_name(chosen_v):  # counters of this same kernel
+            traffic = tj.get("hbm_bytes_per_launch")
 
     # ---- plain FP32 path: fastest FP32 variant, timed like the chosen one
     fp32 = None
@@ -312,6 +339,7 @@ def main():
     if table:
         best = min(range(len(sel.VARIANTS)), key=lambda i: table[i]["ms"])
         selector_info = {"pick": None if pick is None else sel.variant_name(sel.VARIANTS[pick]),
+                         "out_of_training_range": ood,
                          "exhaustive_best": sel.variant_name(sel.VARIANTS[best]),
                          "regret": None if pick is None else round(table[pick]["ms"] / table[best]["ms"] - 1, 4)}
 

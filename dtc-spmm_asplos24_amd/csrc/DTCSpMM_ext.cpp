@@ -416,7 +416,7 @@ py::tuple csr_transpose(Tensor nodePointer, Tensor edgeList, int64_t num_rows, i
   return py::make_tuple(rpt, ct, perm);
 }
 
-Tensor sddmm_csr(Tensor nodePointer, Tensor edgeList, Tensor X, Tensor Y) {
+Tensor sddmm_csr(Tensor nodePointer, Tensor edgeList, Tensor X, Tensor Y, c10::optional<Tensor> edgeToRow) {
   CHECK_INPUT(nodePointer);
   CHECK_INPUT(edgeList);
   CHECK_INPUT(X);
@@ -427,11 +427,18 @@ Tensor sddmm_csr(Tensor nodePointer, Tensor edgeList, Tensor X, Tensor Y) {
   CHECK_DTYPE(Y, torch::kFloat32);
   TORCH_CHECK(X.dim() == 2 && Y.dim() == 2 && X.size(1) == Y.size(1), "X and Y must be 2-D with equal widths");
   TORCH_CHECK(nodePointer.numel() == X.size(0) + 1, "X must have one row per CSR row");
+  const int32_t *e2r = nullptr;
+  if (edgeToRow.has_value()) {
+    CHECK_INPUT(*edgeToRow);
+    CHECK_DTYPE(*edgeToRow, torch::kInt32);
+    TORCH_CHECK(edgeToRow->numel() == edgeList.numel(), "edgeToRow must have one entry per edge");
+    e2r = edgeToRow->data_ptr<int32_t>();
+  }
   const c10::OptionalDeviceGuard guard(X.device());
   Tensor out = torch::empty({edgeList.numel()}, X.options());
-  check_status(dtc_sddmm_csr(nodePointer.data_ptr<int32_t>(), edgeList.data_ptr<int32_t>(), (int32_t)X.size(0),
-                             (int32_t)edgeList.numel(), X.data_ptr<float>(), Y.data_ptr<float>(), (int32_t)Y.size(0),
-                             (int32_t)X.size(1), out.data_ptr<float>(), current_stream()),
+  check_status(dtc_sddmm_csr(nodePointer.data_ptr<int32_t>(), edgeList.data_ptr<int32_t>(), e2r,
+                             (int32_t)X.size(0), (int32_t)edgeList.numel(), X.data_ptr<float>(), Y.data_ptr<float>(),
+                             (int32_t)Y.size(0), (int32_t)X.size(1), out.data_ptr<float>(), current_stream()),
                "sddmm_csr");
   return out;
 }
@@ -718,7 +725,7 @@ PYBIND11_MODULE(DTCSpMM, m) {
   m.def("csr_transpose", &csr_transpose, "CSR transpose on the GPU -> (rowptr_t, col_t, perm)",
         py::arg("nodePointer"), py::arg("edgeList"), py::arg("num_rows"), py::arg("num_cols"));
   m.def("sddmm_csr", &sddmm_csr, "out[e] = X[row(e)] . Y[col(e)]", py::arg("nodePointer"), py::arg("edgeList"),
-        py::arg("X"), py::arg("Y"));
+        py::arg("X"), py::arg("Y"), py::arg("edgeToRow") = py::none());
   m.def("gather", &gather, "out[p] = values[index[p]]", py::arg("values"), py::arg("index"));
   m.attr("BLK_H") = kBlkH;
   m.attr("BLK_W") = kBlkW;

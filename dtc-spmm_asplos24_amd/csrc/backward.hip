@@ -90,72 +90,101 @@ __global__ __launch_bounds__(256) void k_transpose_fill(const uint64_t *skeys,
   if (i <= K) rowptr_t[i] = lower_bound_dev<uint64_t>(skeys, 0, nnz, (uint64_t)i);
 }
 
-// out[e] = x[row(e)] . y[col(e)]: one wavefront per row, four 16-lane groups
-// each taking one edge at a time, VEC floats per lane per step.
+// out[e] = x[row(e)] . y[col(e)], edge-parallel: every wavefront takes 8
+// consecutive edges at a time (CSR order), all 64 lanes reading VEC floats of
+// each edge's two rows (one 1 KB row per edge at n = 256 and VEC = 4), so a
+// hub row is spread over many wavefronts.  The row of an edge comes from
+// edge_to_row when given, else from a binary search of rowptr (lanes 0-7,
+// one edge each).  The 8 partial dot products are reduced together: three
+// halving exchanges (4, 2, 1 values) then three plain butterfly steps.
+constexpr int kSddmmEdges = 8;
+
 template <int VEC>
 __global__ __launch_bounds__(256) void k_sddmm(const int32_t *rowptr, const int32_t *col,
-                                               int32_t M, const float *x, const float *y,
-                                               int32_t y_rows, int32_t n, float *out) {
+                                               const int32_t *e2r, int32_t M, int32_t nnz,
+                                               const float *x, const float *y, int32_t y_rows,
+                                               int32_t n, float *out) {
   typedef float vfloat __attribute__((ext_vector_type(VEC)));
-  const int lane = threadIdx.x & 63;
-  const int grp = lane >> 4;
-  const int q = lane & 15;
+  const int lane = lane_id();
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  for (int64_t r = wave; r < M; r += nwaves) {
-    const int s = rowptr[r];
-    const int e = rowptr[r + 1];
-    const float *xr = x + r * (int64_t)n;
-    for (int base = s; base < e; base += 4) {
-      const int ed = base + grp;
-      float acc = 0.f;
-      if (ed < e) {
-        const int j = col[ed];
-        if ((unsigned)j < (unsigned)y_rows) {
-          const float *yr = y + (int64_t)j * n;
-          for (int c = q * VEC; c < n; c += 16 * VEC) {
-            const vfloat xv = *reinterpret_cast<const vfloat *>(xr + c);
-            const vfloat yv = *reinterpret_cast<const vfloat *>(yr + c);
-#pragma unroll
-            for (int v = 0; v < VEC; ++v) acc += xv[v] * yv[v];
-          }
+  const int64_t groups = ((int64_t)nnz + kSddmmEdges - 1) / kSddmmEdges;
+  for (int64_t g = wave; g < groups; g += nwaves) {
+    const int64_t e0 = g * kSddmmEdges;
+    // rows and columns of the 8 edges (lane k holds edge e0 + k)
+    int my_row = 0, my_col = -1;
+    if (lane < kSddmmEdges && e0 + lane < nnz) {
+      const int e = (int)(e0 + lane);
+      my_col = col[e];
+      if (e2r) {
+        my_row = e2r[e];
+      } else {
+        int lo = 0, hi = M;  // last row r with rowptr[r] <= e
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (rowptr[mid] <= e) lo = mid;
+          else hi = mid;
         }
+        my_row = lo;
+      }
+      if ((unsigned)my_row >= (unsigned)M) my_col = -1;  // defensive: never read outside x
+    }
+    // Branch-free loads: an edge without a valid column reads row 0 of x and
+    // y (both exist: M >= 1 and y_rows >= 1 here) and is masked afterwards,
+    // so all 16 row loads of a lane are in flight together.
+    int rk[kSddmmEdges], jk[kSddmmEdges];
+    bool ok[kSddmmEdges];
+#pragma unroll
+    for (int k = 0; k < kSddmmEdges; ++k) {
+      rk[k] = __builtin_amdgcn_readlane(my_row, k);
+      jk[k] = __builtin_amdgcn_readlane(my_col, k);
+      ok[k] = (unsigned)jk[k] < (unsigned)y_rows;
+      if (!ok[k]) rk[k] = jk[k] = 0;
+    }
+    float p[kSddmmEdges];
+#pragma unroll
+    for (int k = 0; k < kSddmmEdges; ++k) p[k] = 0.f;
+    for (int c = lane * VEC; c < n; c += kWave * VEC) {
+      vfloat xv[kSddmmEdges], yv[kSddmmEdges];
+#pragma unroll
+      for (int k = 0; k < kSddmmEdges; ++k) {
+        xv[k] = *reinterpret_cast<const vfloat *>(x + (int64_t)rk[k] * n + c);
+        yv[k] = *reinterpret_cast<const vfloat *>(y + (int64_t)jk[k] * n + c);
       }
 #pragma unroll
-      for (int d = 8; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, 64);
-      if (q == 0 && ed < e) out[ed] = acc;
+      for (int k = 0; k < kSddmmEdges; ++k)
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) p[k] += xv[k][v] * yv[k][v];
     }
+#pragma unroll
+    for (int k = 0; k < kSddmmEdges; ++k)
+      if (!ok[k]) p[k] = 0.f;
+    // halving exchanges: lanes keep the half of the values their bit selects
+    const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8;
+    float q[4], t[2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float send = b5 ? p[i] : p[i + 4];
+      q[i] = (b5 ? p[i + 4] : p[i]) + __shfl_xor(send, 32, kWave);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const float send = b4 ? q[i] : q[i + 2];
+      t[i] = (b4 ? q[i + 2] : q[i]) + __shfl_xor(send, 16, kWave);
+    }
+    float v = (b3 ? t[1] : t[0]) + __shfl_xor(b3 ? t[0] : t[1], 8, kWave);
+    v += __shfl_xor(v, 4, kWave);
+    v += __shfl_xor(v, 2, kWave);
+    v += __shfl_xor(v, 1, kWave);
+    const int k = (lane >> 3) & 7;  // edge index this lane's sum belongs to
+    if ((lane & 7) == 0 && e0 + k < nnz) out[e0 + k] = v;
   }
 }
 
-template <>
-__global__ __launch_bounds__(256) void k_sddmm<1>(const int32_t *rowptr, const int32_t *col,
-                                                  int32_t M, const float *x, const float *y,
-                                                  int32_t y_rows, int32_t n, float *out) {
-  const int lane = threadIdx.x & 63;
-  const int grp = lane >> 4;
-  const int q = lane & 15;
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  for (int64_t r = wave; r < M; r += nwaves) {
-    const int s = rowptr[r];
-    const int e = rowptr[r + 1];
-    const float *xr = x + r * (int64_t)n;
-    for (int base = s; base < e; base += 4) {
-      const int ed = base + grp;
-      float acc = 0.f;
-      if (ed < e) {
-        const int j = col[ed];
-        if ((unsigned)j < (unsigned)y_rows) {
-          const float *yr = y + (int64_t)j * n;
-          for (int c = q; c < n; c += 16) acc += xr[c] * yr[c];
-        }
-      }
-#pragma unroll
-      for (int d = 8; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, 64);
-      if (q == 0 && ed < e) out[ed] = acc;
-    }
-  }
+__global__ void k_zero_f32(float *p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0.f;
 }
 
 __global__ __launch_bounds__(256) void k_gather(const float *in, const int32_t *idx,
@@ -228,8 +257,9 @@ dtc_status_t dtc_csr_transpose(const int32_t *node_pointer, const int32_t *edge_
 }
 
 dtc_status_t dtc_sddmm_csr(const int32_t *node_pointer, const int32_t *edge_list,
-                           int32_t num_rows, int32_t nnz, const float *x, const float *y,
-                           int32_t y_rows, int32_t n, float *out, dtc_stream_t stream_) {
+                           const int32_t *edge_to_row, int32_t num_rows, int32_t nnz,
+                           const float *x, const float *y, int32_t y_rows, int32_t n, float *out,
+                           dtc_stream_t stream_) {
   clear_error();
   hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
   if (num_rows < 0 || nnz < 0 || y_rows < 0 || n <= 0)
@@ -237,12 +267,20 @@ dtc_status_t dtc_sddmm_csr(const int32_t *node_pointer, const int32_t *edge_list
   if (!node_pointer) return fail(DTC_ERROR_INVALID_VALUE, "node_pointer is NULL");
   if (nnz == 0 || num_rows == 0) return DTC_SUCCESS;
   if (!edge_list || !x || !y || !out) return fail(DTC_ERROR_INVALID_VALUE, "NULL argument");
-  const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(65536, ceil_div(num_rows, 4)));
+  if (y_rows == 0) {  // every column is out of range: all zeros
+    k_zero_f32<<<(unsigned)std::min<int64_t>(4096, ceil_div(nnz, 256)), 256, 0, stream>>>(out, nnz);
+    DTC_LAUNCH_CHECK("zero");
+    return DTC_SUCCESS;
+  }
+  const int64_t groups = ceil_div(nnz, kSddmmEdges);
+  const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(1 << 16, ceil_div(groups, 4)));
   const bool v4 = n % 4 == 0 && (uintptr_t)x % 16 == 0 && (uintptr_t)y % 16 == 0;
   if (v4)
-    k_sddmm<4><<<grid, 256, 0, stream>>>(node_pointer, edge_list, num_rows, x, y, y_rows, n, out);
+    k_sddmm<4><<<grid, 256, 0, stream>>>(node_pointer, edge_list, edge_to_row, num_rows, nnz, x, y,
+                                         y_rows, n, out);
   else
-    k_sddmm<1><<<grid, 256, 0, stream>>>(node_pointer, edge_list, num_rows, x, y, y_rows, n, out);
+    k_sddmm<1><<<grid, 256, 0, stream>>>(node_pointer, edge_list, edge_to_row, num_rows, nnz, x, y,
+                                         y_rows, n, out);
   DTC_LAUNCH_CHECK("sddmm");
   return DTC_SUCCESS;
 }

@@ -625,7 +625,7 @@ dtc_status_t resolve_shape(const dtc_spmm_options_t *opt, int32_t n, const void 
     auto fits = [&](int v) {
       return n % v == 0 && ((uintptr_t)b % (2 * v)) == 0 && ((uintptr_t)c % (4 * v)) == 0;
     };
-    if (vec == 0) vec = fits(8) ? 8 : 4;
+    if (vec == 0) vec = 4;  // 64-column tiles: measured faster than 128 (more waves in flight)
     if (!fits(vec))
       return fail(DTC_ERROR_INVALID_VALUE,
                   "bfloat16 B needs N to be a multiple of 4 and 8-byte aligned B/C rows");
@@ -645,7 +645,7 @@ dtc_status_t resolve_shape(const dtc_spmm_options_t *opt, int32_t n, const void 
   if (!fits(vec))
     return fail(DTC_ERROR_INVALID_VALUE,
                 "vec " + std::to_string(vec) + " needs N and the B/C base addresses to be multiples of it");
-  if (nsub == 0) nsub = (vec == 4 && n >= 128) ? 2 : 1;
+  if (nsub == 0) nsub = 1;  // one 16*vec column tile per wave measured fastest on R-MAT and SBM
   if (nsub == 2 && vec != 4) nsub = 1;
   out->prec = prec;
   out->vec = vec;
@@ -775,7 +775,7 @@ dtc_status_t dtc_spmm_balanced(const int32_t *tcblock_rowid, const uint8_t *tcbl
   Variant var;
   st = resolve_shape(options, n, b, c, &var);
   if (st != DTC_SUCCESS) return st;
-  const int chunk = (options && options->chunk_blocks > 0) ? options->chunk_blocks : 64;
+  const int chunk = (options && options->chunk_blocks > 0) ? options->chunk_blocks : 256;
   const bool persistent = options && options->persistent;
   if (persistent && (!workspace || workspace_bytes < 256))
     return fail(DTC_ERROR_WORKSPACE, "persistent balanced SpMM needs a 256-byte workspace");

@@ -6,7 +6,7 @@ accumulation, bf16 result); its bf16 output is gathered by the SpMM as is
 (b_dtype bfloat16: half the bytes of a float32 B), with A's values split into
 three bf16 limbs and float32 accumulation.  The backward pass:
     dXW     = A^T dH            (ME-TCF of A^T, float32 SpMM)
-    dX      = dXW W^T,  dW = X^T dXW   (dense GEMMs)
+    dX      = dXW W^T,  dW = X^T dXW   (dense GEMMs; dW split over K)
     dvalues = SDDMM(dH, XW)     on A's pattern.
 The reference's GCN entry (`DTCSpMM_gcn`, DTCSpMM_kernel.cu:1033-1116) is a
 forward only and its "backward" names return A * dY (DTCSpMM.cpp:384-399).
@@ -17,6 +17,19 @@ import torch
 
 from . import load_extension
 from .matrix import DTCMatrix
+
+
+def _xt_dy(X: torch.Tensor, dY: torch.Tensor, parts: int = 64) -> torch.Tensor:
+    """X^T dY for tall X [K x F] and dY [K x G]: the product has only F x G
+    outputs (one or two GEMM tiles at 256 x 256), so K is split into `parts`
+    batches that run side by side and their partial products are summed in
+    float32.  Both bmm operands are made contiguous first."""
+    K = X.shape[0]
+    if K % parts or K // parts < 512:
+        return torch.matmul(X.t().contiguous(), dY.contiguous())
+    Xs = X.reshape(parts, K // parts, X.shape[1]).transpose(1, 2).contiguous()
+    Ys = dY.contiguous().reshape(parts, K // parts, dY.shape[1])
+    return torch.bmm(Xs, Ys).float().sum(0).to(X.dtype)
 
 
 class _GCNFunction(torch.autograd.Function):
@@ -39,7 +52,7 @@ class _GCNFunction(torch.autograd.Function):
         dH = dH.contiguous().float()
         d_values = d_X = d_W = None
         if ctx.needs_input_grad[0]:
-            d_values = ext.sddmm_csr(A.rowptr, A.col, dH, XW.float().contiguous())
+            d_values = ext.sddmm_csr(A.rowptr, A.col, dH, XW.float().contiguous(), A.edge_to_row)
         if ctx.needs_input_grad[1] or ctx.needs_input_grad[2]:
             At = A.transpose()
             vals_t_tc = ext.gather(ext.gather(values.contiguous(), A.perm_t), At.tc_edge)
@@ -48,7 +61,7 @@ class _GCNFunction(torch.autograd.Function):
             if ctx.needs_input_grad[1]:
                 d_X = torch.matmul(dXW, W.t().contiguous())
             if ctx.needs_input_grad[2]:
-                d_W = torch.matmul(X.t().contiguous(), dXW)
+                d_W = _xt_dy(X, dXW)
         return d_values, d_X, d_W, None, None
 
 

@@ -116,7 +116,7 @@ class _SpMMFunction(torch.autograd.Function):
         dC = dC.contiguous()
         d_values = d_B = None
         if ctx.needs_input_grad[0]:
-            d_values = ext.sddmm_csr(A.rowptr, A.col, dC, B.contiguous())
+            d_values = ext.sddmm_csr(A.rowptr, A.col, dC, B.contiguous(), A.edge_to_row)
         if ctx.needs_input_grad[1]:
             At = A.transpose()
             vals_t_csr = ext.gather(values.contiguous(), A.perm_t)

@@ -64,6 +64,26 @@ FEATURE_NAMES = [
     "has_values",
 ]
 
+# Scale features whose training range bounds where the network is trusted:
+# a matrix more than RANGE_SLACK (log2 units, x1.41) outside the training
+# extremes of any of them goes to exhaustive timing instead.
+RANGE_CHECKED = ["log2_rows", "log2_nnz", "log2_N", "log2_avg_deg", "log2_max_deg", "log2_blocks",
+                 "log2_mean_blocks_per_window", "log2_max_blocks_per_window", "log2_B_bytes"]
+RANGE_SLACK = 0.5
+
+
+def out_of_range(x, lo, hi) -> list:
+    """Names of the range-checked features of x outside [lo - slack, hi + slack]."""
+    if lo is None or hi is None:
+        return []
+    x = np.asarray(x, np.float64).reshape(-1)
+    bad = []
+    for name in RANGE_CHECKED:
+        i = FEATURE_NAMES.index(name)
+        if x[i] < lo[i] - RANGE_SLACK or x[i] > hi[i] + RANGE_SLACK:
+            bad.append(name)
+    return bad
+
 
 def variant_name(v) -> str:
     s, p, c, vec, sub, pers = v
@@ -219,6 +239,7 @@ class MLPSelector:
             self.params.append([torch.from_numpy(w).cuda(), torch.from_numpy(b).cuda()])
         self.mu = np.zeros(in_dim, np.float32)
         self.sigma = np.ones(in_dim, np.float32)
+        self.lo = self.hi = None  # training feature range
 
     # --- helpers
     def _norm(self, X):
@@ -251,6 +272,7 @@ class MLPSelector:
         X = np.asarray(X, np.float32)
         self.mu = X.mean(0)
         self.sigma = X.std(0) + 1e-6
+        self.lo, self.hi = X.min(0), X.max(0)
         Xd = self._norm(X)
         tgt, mask = self.targets(times_ms)
         Td = torch.from_numpy(tgt).cuda()
@@ -279,11 +301,15 @@ class MLPSelector:
     def choose(self, X) -> np.ndarray:
         return np.argmin(self.predict(X), axis=1)
 
+    def out_of_range(self, x) -> list:
+        return out_of_range(x, self.lo, self.hi)
+
     def save(self, path: str):
         flat = {f"w{i}": p[0].cpu().numpy() for i, p in enumerate(self.params)}
         flat.update({f"b{i}": p[1].cpu().numpy() for i, p in enumerate(self.params)})
+        extra = {} if self.lo is None else {"lo": self.lo, "hi": self.hi}
         np.savez(path, mu=self.mu, sigma=self.sigma, variants=np.array([variant_name(v) for v in VARIANTS]),
-                 **flat)
+                 **extra, **flat)
 
     @classmethod
     def load(cls, path: str = DEFAULT_MODEL) -> "MLPSelector":
@@ -296,7 +322,21 @@ class MLPSelector:
         obj.params = [[torch.from_numpy(z[f"w{i}"]).cuda(), torch.from_numpy(z[f"b{i}"]).cuda()]
                       for i in range(3)]
         obj.mu, obj.sigma = z["mu"], z["sigma"]
+        obj.lo = z["lo"] if "lo" in z else None
+        obj.hi = z["hi"] if "hi" in z else None
         return obj
+
+
+def pick_variant(A, B, model: MLPSelector | None = None):
+    """(variant index, how): the network's pick when A lies inside its
+    training range, else the fastest variant by exhaustive timing."""
+    N = int(B.shape[1])
+    x = features(A, N)
+    bad = ["no model"] if model is None else model.out_of_range(x)
+    if not bad:
+        return int(model.choose(x[None, :])[0]), "selector"
+    t = time_variants(A, B)
+    return int(np.nanargmin(t)), "exhaustive (" + ", ".join(bad) + " outside the training range)"
 
 
 def regret(times_ms, chosen) -> np.ndarray:
@@ -329,7 +369,11 @@ def evaluate(model: MLPSelector, data, holdout_from: int = 1600) -> dict:
     r_def = regret(t, np.full(t.shape[0], DEFAULT_VARIANT))
     r_fix = regret(t, np.full(t.shape[0], fixed))
     picked = t[np.arange(t.shape[0]), chosen]
+    fallback = np.array([bool(model.out_of_range(x)) for x in X])
+    r_guard = np.where(fallback, 0.0, r_sel)  # exhaustive timing has no regret
     return {"heldout_matrices": int(test.sum()),
+            "fallback_to_exhaustive": int(fallback.sum()),
+            "guarded_mean_regret": float(r_guard.mean()),
             "selector_mean_regret": float(r_sel.mean()), "selector_p90_regret": float(np.quantile(r_sel, 0.9)),
             "default_variant": variant_name(VARIANTS[DEFAULT_VARIANT]),
             "default_mean_regret": float(r_def.mean()), "default_p90_regret": float(np.quantile(r_def, 0.9)),

@@ -52,9 +52,10 @@ typedef enum dtc_b_dtype {
 typedef struct dtc_spmm_options {
   int32_t precision;    /* dtc_precision_t */
   int32_t vec;          /* B elements per lane per gathered row: 1, 2, 4 (float32 B);
-                           4, 8 (bfloat16 B); 0 = widest N and alignment allow */
-  int32_t subtiles;     /* 1 or 2 column tiles of 16*vec per wave (0 = auto; float32 B) */
-  int32_t chunk_blocks; /* balanced schedule: TC blocks per work chunk (0 = 64) */
+                           4, 8 (bfloat16 B); 0 = the widest float32 width N and
+                           alignment allow, 4 for bfloat16 B */
+  int32_t subtiles;     /* 1 or 2 column tiles of 16*vec per wave (0 = 1; float32 B) */
+  int32_t chunk_blocks; /* balanced schedule: TC blocks per work chunk (0 = 256) */
   int32_t persistent;   /* balanced schedule: 1 = persistent waves pulling chunks from a queue */
   int32_t b_dtype;      /* dtc_b_dtype_t */
 } dtc_spmm_options_t;
@@ -186,10 +187,13 @@ dtc_status_t dtc_csr_transpose(const int32_t *node_pointer, const int32_t *edge_
                                void *workspace, size_t workspace_bytes, dtc_stream_t stream);
 
 /* out[e] = sum_j x[row(e), j] * y[col(e), j] for every CSR edge e
- * (x: [num_rows x n], y: [y_rows x n]; columns >= y_rows give 0). */
+ * (x: [num_rows x n], y: [y_rows x n]; columns >= y_rows give 0).
+ * edge_to_row (the edgeToRow output of the ME-TCF build) may be NULL: rows
+ * are then found by binary search of node_pointer. */
 dtc_status_t dtc_sddmm_csr(const int32_t *node_pointer, const int32_t *edge_list,
-                           int32_t num_rows, int32_t nnz, const float *x, const float *y,
-                           int32_t y_rows, int32_t n, float *out, dtc_stream_t stream);
+                           const int32_t *edge_to_row, int32_t num_rows, int32_t nnz,
+                           const float *x, const float *y, int32_t y_rows, int32_t n, float *out,
+                           dtc_stream_t stream);
 
 /* out[p] = in[idx[p]] for p < count (moves values between CSR and TC-block order). */
 dtc_status_t dtc_gather_f32(const float *in, const int32_t *idx, int32_t count, float *out,

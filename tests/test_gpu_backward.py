@@ -30,16 +30,29 @@ def test_transpose_vs_oracle(ext):
         assert np.array_equal(perm.cpu().numpy(), p2)
 
 
-@pytest.mark.parametrize("N", [1, 3, 16, 64, 130])
-def test_sddmm_vs_oracle(ext, N):
+@pytest.mark.parametrize("N", [1, 3, 16, 64, 130, 256, 300, 512])
+@pytest.mark.parametrize("with_rows", [False, True])
+def test_sddmm_vs_oracle(ext, N, with_rows):
+    """Edge-parallel SDDMM vs float64, rows from edgeToRow or from the row
+    pointer's binary search; a 3,000-entry hub row, empty rows, nnz not a
+    multiple of 8."""
     rng = np.random.default_rng(N)
-    M, K = 700, 500
+    M, K = 700, 3500
     rowptr, col = random_csr(rng, M, K, 8.0, empty_frac=0.1)
+    hub = np.sort(rng.choice(K, 3000, replace=False)).astype(np.int32)  # row 5 becomes a hub
+    cols = [col[rowptr[r]:rowptr[r + 1]] for r in range(M)]
+    cols[5] = hub
+    if sum(len(c) for c in cols) % 8 == 0:
+        cols[5] = hub[:-1]
+    rowptr = np.concatenate([[0], np.cumsum([len(c) for c in cols])]).astype(np.int32)
+    col = np.concatenate(cols).astype(np.int32)
     X = rng.standard_normal((M, N)).astype(np.float32)
     Y = rng.standard_normal((K, N)).astype(np.float32)
-    say("sddmm", N)
+    say("sddmm", N, with_rows)
+    e2r = torch.from_numpy(np.repeat(np.arange(M), np.diff(rowptr)).astype(np.int32)).cuda()
     out = ext.sddmm_csr(torch.from_numpy(rowptr).cuda(), torch.from_numpy(col).cuda(),
-                        torch.from_numpy(X).cuda(), torch.from_numpy(Y).cuda()).cpu().numpy()
+                        torch.from_numpy(X).cuda(), torch.from_numpy(Y).cuda(),
+                        e2r if with_rows else None).cpu().numpy()
     ref = oracle.sddmm_csr_f64(rowptr, col, X, Y, M)
     rows = np.repeat(np.arange(M), np.diff(rowptr))
     scale = (np.abs(X[rows]) * np.abs(Y[col])).sum(1)

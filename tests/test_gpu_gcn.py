@@ -68,7 +68,8 @@ def _dense_reference(a, values, X, W, dH):
 @pytest.mark.parametrize("schedule", ["windows", "balanced"])
 def test_gcn_layer_rmat_2048(ext, schedule):
     """R-MAT scale 11 (2,048 nodes), symmetric normalisation, X 2048x256 and
-    W 256x256 in bf16 (Xavier), gradients vs float64."""
+    W 256x256 in bf16 (Xavier), gradients vs float64 autograd and float64
+    finite differences."""
     from dtc.gcn import gcn_layer
     from dtc.graphs import rmat
     from dtc.matrix import DTCMatrix
@@ -95,3 +96,30 @@ def test_gcn_layer_rmat_2048(ext, schedule):
     assert rel(v.grad.cpu().numpy(), gv64) < 2e-2
     assert rel(Xc.grad.float().cpu().numpy(), gX64) < 2e-2
     assert rel(Wc.grad.float().cpu().numpy(), gW64) < 2e-2
+
+    # float64 central finite differences of L = <H, dH> along random
+    # directions of values, X and W, against the GPU gradients' projections
+    # (the loss is bilinear per argument, so the difference quotient is exact
+    # up to float64 rounding; the bf16 forward sets the 2e-2 tolerance).
+    rows = np.repeat(np.arange(M), np.diff(a.rowptr))
+    Xd, Wd, dHd = X.double().numpy(), W.double().numpy(), dH.astype(np.float64)
+
+    def loss(vals, Xm, Wm):
+        XW = Xm @ Wm
+        Hm = np.zeros((M, F))
+        np.add.at(Hm, rows, vals[:, None] * XW[a.col])
+        return float((Hm * dHd).sum())
+
+    rng2 = np.random.default_rng(11)
+    base = (a.values.astype(np.float64), Xd, Wd)
+    grads = (v.grad.double().cpu().numpy(), Xc.grad.double().cpu().numpy(), Wc.grad.double().cpu().numpy())
+    for i in range(3):
+        d = rng2.standard_normal(base[i].shape)
+        eps = 1e-3 * np.abs(base[i]).max() / np.abs(d).max()
+        plus = list(base)
+        minus = list(base)
+        plus[i] = base[i] + eps * d
+        minus[i] = base[i] - eps * d
+        fd = (loss(*plus) - loss(*minus)) / (2 * eps)
+        terms = grads[i] * d  # the projection sums these; its rounding grows like their 2-norm
+        assert abs(fd - float(terms.sum())) <= 2e-2 * (abs(fd) + float(np.sqrt((terms ** 2).sum()))), i

@@ -62,3 +62,23 @@ def test_dataset_and_model_consistent():
         m = np.load(DEFAULT_MODEL)
         assert [str(s) for s in m["variants"]] == [variant_name(v) for v in VARIANTS]
         assert m["w0"].shape[0] == len(FEATURE_NAMES) and m["w2"].shape[1] == len(VARIANTS)
+
+
+def test_model_range_guard():
+    """The shipped network carries its training feature range; the 232,965-row
+    block model (121M entries, 10.4M TC blocks) lies outside it and goes to
+    exhaustive timing, while the training matrices and all but a few of the
+    400 held-out ones lie inside."""
+    from dtc import selector as sel
+    z = np.load(sel.DEFAULT_MODEL)
+    d = np.load(sel.DEFAULT_DATA)
+    train = d["seed"] < 1600
+    X = d["features"].astype(np.float32)
+    assert np.array_equal(z["lo"], X[train].min(0)) and np.array_equal(z["hi"], X[train].max(0))
+    assert all(sel.out_of_range(x, z["lo"], z["hi"]) == [] for x in X[train])
+    assert sum(1 for x in X[~train] if sel.out_of_range(x, z["lo"], z["hi"])) <= 4
+    x = X[0].copy()
+    x[sel.FEATURE_NAMES.index("log2_nnz")] = np.log2(1.21e8)
+    x[sel.FEATURE_NAMES.index("log2_blocks")] = np.log2(1.04e7)
+    assert sel.out_of_range(x, z["lo"], z["hi"]) == ["log2_nnz", "log2_blocks"]
+    assert sel.out_of_range(x, None, None) == []
