@@ -424,7 +424,8 @@ Tensor sddmm_csr(Tensor nodePointer, Tensor edgeList, Tensor X, Tensor Y, c10::o
   CHECK_DTYPE(nodePointer, torch::kInt32);
   CHECK_DTYPE(edgeList, torch::kInt32);
   CHECK_DTYPE(X, torch::kFloat32);
-  CHECK_DTYPE(Y, torch::kFloat32);
+  TORCH_CHECK(Y.scalar_type() == torch::kFloat32 || Y.scalar_type() == torch::kBFloat16,
+              "Y must be float32 or bfloat16, got ", Y.scalar_type());
   TORCH_CHECK(X.dim() == 2 && Y.dim() == 2 && X.size(1) == Y.size(1), "X and Y must be 2-D with equal widths");
   TORCH_CHECK(nodePointer.numel() == X.size(0) + 1, "X must have one row per CSR row");
   const int32_t *e2r = nullptr;
@@ -436,9 +437,11 @@ Tensor sddmm_csr(Tensor nodePointer, Tensor edgeList, Tensor X, Tensor Y, c10::o
   }
   const c10::OptionalDeviceGuard guard(X.device());
   Tensor out = torch::empty({edgeList.numel()}, X.options());
+  const int32_t y_dtype = Y.scalar_type() == torch::kBFloat16 ? DTC_B_BFLOAT16 : DTC_B_FLOAT32;
   check_status(dtc_sddmm_csr(nodePointer.data_ptr<int32_t>(), edgeList.data_ptr<int32_t>(), e2r,
-                             (int32_t)X.size(0), (int32_t)edgeList.numel(), X.data_ptr<float>(), Y.data_ptr<float>(),
-                             (int32_t)Y.size(0), (int32_t)X.size(1), out.data_ptr<float>(), current_stream()),
+                             (int32_t)X.size(0), (int32_t)edgeList.numel(), X.data_ptr<float>(), Y.data_ptr(),
+                             y_dtype, (int32_t)Y.size(0), (int32_t)X.size(1), out.data_ptr<float>(),
+                             current_stream()),
                "sddmm_csr");
   return out;
 }

@@ -8,6 +8,7 @@
 //   dA = SDDMM     -> dA[e] = dC[row(e), :] . B[col(e), :] on A's pattern.
 #include <algorithm>
 #include <string>
+#include <type_traits>
 
 #include "dtc_internal.h"
 
@@ -99,12 +100,33 @@ __global__ __launch_bounds__(256) void k_transpose_fill(const uint64_t *skeys,
 // halving exchanges (4, 2, 1 values) then three plain butterfly steps.
 constexpr int kSddmmEdges = 8;
 
+// Y as bfloat16 (YBF = true): VEC bf16 per lane (8 bytes at VEC = 4), taken
+// exactly and widened to float32 before the product.
+template <int VEC, bool YBF>
+struct YVec;
 template <int VEC>
+struct YVec<VEC, false> {
+  typedef float type __attribute__((ext_vector_type(VEC)));
+  __device__ static float at(const type &v, int i) { return v[i]; }
+};
+template <int VEC>
+struct YVec<VEC, true> {
+  typedef unsigned short type __attribute__((ext_vector_type(VEC)));
+  __device__ static float at(const type &v, int i) {
+    return __uint_as_float((uint32_t)v[i] << 16);
+  }
+};
+
+template <int VEC, bool YBF>
 __global__ __launch_bounds__(256) void k_sddmm(const int32_t *rowptr, const int32_t *col,
                                                const int32_t *e2r, int32_t M, int32_t nnz,
-                                               const float *x, const float *y, int32_t y_rows,
+                                               const float *x, const void *y_, int32_t y_rows,
                                                int32_t n, float *out) {
   typedef float vfloat __attribute__((ext_vector_type(VEC)));
+  typedef YVec<VEC, YBF> YV;
+  typedef typename YV::type yvec;
+  typedef typename std::conditional<YBF, unsigned short, float>::type yelem;
+  const yelem *y = static_cast<const yelem *>(y_);
   const int lane = lane_id();
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
@@ -145,16 +167,17 @@ __global__ __launch_bounds__(256) void k_sddmm(const int32_t *rowptr, const int3
 #pragma unroll
     for (int k = 0; k < kSddmmEdges; ++k) p[k] = 0.f;
     for (int c = lane * VEC; c < n; c += kWave * VEC) {
-      vfloat xv[kSddmmEdges], yv[kSddmmEdges];
+      vfloat xv[kSddmmEdges];
+      yvec yv[kSddmmEdges];
 #pragma unroll
       for (int k = 0; k < kSddmmEdges; ++k) {
         xv[k] = *reinterpret_cast<const vfloat *>(x + (int64_t)rk[k] * n + c);
-        yv[k] = *reinterpret_cast<const vfloat *>(y + (int64_t)jk[k] * n + c);
+        yv[k] = *reinterpret_cast<const yvec *>(y + (int64_t)jk[k] * n + c);
       }
 #pragma unroll
       for (int k = 0; k < kSddmmEdges; ++k)
 #pragma unroll
-        for (int v = 0; v < VEC; ++v) p[k] += xv[k][v] * yv[k][v];
+        for (int v = 0; v < VEC; ++v) p[k] += xv[k][v] * YV::at(yv[k], v);
     }
 #pragma unroll
     for (int k = 0; k < kSddmmEdges; ++k)
@@ -258,12 +281,14 @@ dtc_status_t dtc_csr_transpose(const int32_t *node_pointer, const int32_t *edge_
 
 dtc_status_t dtc_sddmm_csr(const int32_t *node_pointer, const int32_t *edge_list,
                            const int32_t *edge_to_row, int32_t num_rows, int32_t nnz,
-                           const float *x, const float *y, int32_t y_rows, int32_t n, float *out,
-                           dtc_stream_t stream_) {
+                           const float *x, const void *y, int32_t y_dtype, int32_t y_rows,
+                           int32_t n, float *out, dtc_stream_t stream_) {
   clear_error();
   hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
   if (num_rows < 0 || nnz < 0 || y_rows < 0 || n <= 0)
     return fail(DTC_ERROR_INVALID_VALUE, "sizes must be >= 0 and N >= 1");
+  if (y_dtype != DTC_B_FLOAT32 && y_dtype != DTC_B_BFLOAT16)
+    return fail(DTC_ERROR_INVALID_VALUE, "y_dtype must be DTC_B_FLOAT32 or DTC_B_BFLOAT16");
   if (!node_pointer) return fail(DTC_ERROR_INVALID_VALUE, "node_pointer is NULL");
   if (nnz == 0 || num_rows == 0) return DTC_SUCCESS;
   if (!edge_list || !x || !y || !out) return fail(DTC_ERROR_INVALID_VALUE, "NULL argument");
@@ -274,13 +299,20 @@ dtc_status_t dtc_sddmm_csr(const int32_t *node_pointer, const int32_t *edge_list
   }
   const int64_t groups = ceil_div(nnz, kSddmmEdges);
   const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(1 << 16, ceil_div(groups, 4)));
-  const bool v4 = n % 4 == 0 && (uintptr_t)x % 16 == 0 && (uintptr_t)y % 16 == 0;
-  if (v4)
-    k_sddmm<4><<<grid, 256, 0, stream>>>(node_pointer, edge_list, edge_to_row, num_rows, nnz, x, y,
-                                         y_rows, n, out);
+  const bool ybf = y_dtype == DTC_B_BFLOAT16;
+  const bool v4 = n % 4 == 0 && (uintptr_t)x % 16 == 0 && (uintptr_t)y % (ybf ? 8 : 16) == 0;
+  if (v4 && ybf)
+    k_sddmm<4, true><<<grid, 256, 0, stream>>>(node_pointer, edge_list, edge_to_row, num_rows, nnz,
+                                               x, y, y_rows, n, out);
+  else if (v4)
+    k_sddmm<4, false><<<grid, 256, 0, stream>>>(node_pointer, edge_list, edge_to_row, num_rows, nnz,
+                                                x, y, y_rows, n, out);
+  else if (ybf)
+    k_sddmm<1, true><<<grid, 256, 0, stream>>>(node_pointer, edge_list, edge_to_row, num_rows, nnz,
+                                               x, y, y_rows, n, out);
   else
-    k_sddmm<1><<<grid, 256, 0, stream>>>(node_pointer, edge_list, edge_to_row, num_rows, nnz, x, y,
-                                         y_rows, n, out);
+    k_sddmm<1, false><<<grid, 256, 0, stream>>>(node_pointer, edge_list, edge_to_row, num_rows, nnz,
+                                                x, y, y_rows, n, out);
   DTC_LAUNCH_CHECK("sddmm");
   return DTC_SUCCESS;
 }

@@ -17,7 +17,7 @@
 // re-queues stale pairs (TCA_reorder.py:152-187, :250-286) -- a sequential
 // priority-queue algorithm, as in the reference, kept in C++ here.
 #include <algorithm>
-#include <unordered_set>
+#include <queue>
 #include <vector>
 
 #include "dtc_internal.h"
@@ -75,11 +75,15 @@ __global__ void k_zero_u64(unsigned long long *p) {
 
 // flags[0]: CSR faults; flags[1]: largest column.  One wavefront per row:
 // columns must be strictly increasing inside a row (sorted, no duplicates).
+// Results are combined per workgroup first, so the grid issues one atomic pair
+// per workgroup rather than per wavefront.
 __global__ __launch_bounds__(256) void k_check_sorted_csr(const int32_t *rowptr,
                                                          const int32_t *col, int32_t M,
                                                          int32_t nnz, int32_t *flags) {
+  __shared__ int s_f[4], s_m[4];
   const int lane = lane_id();
-  const int64_t first = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int wave = threadIdx.x >> 6;
+  const int64_t first = (int64_t)blockIdx.x * 4 + wave;
   const int64_t stride = (int64_t)gridDim.x * 4;
   int f = 0, mx = 0;
   if (first == 0 && lane == 0) {
@@ -102,36 +106,67 @@ __global__ __launch_bounds__(256) void k_check_sorted_csr(const int32_t *rowptr,
   for (int d = kWave / 2; d >= 1; d >>= 1) f |= __shfl_xor(f, d, kWave);
   mx = wave_max(mx);
   if (lane == 0) {
-    if (f) atomicOr(&flags[0], f);
-    atomicMax(&flags[1], mx);
+    s_f[wave] = f;
+    s_m[wave] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int bf = s_f[0] | s_f[1] | s_f[2] | s_f[3];
+    const int bm = max(max(s_m[0], s_m[1]), max(s_m[2], s_m[3]));
+    if (bf) atomicOr(&flags[0], bf);
+    atomicMax(&flags[1], bm);
   }
 }
 
-// One wavefront per row; lane j holds hashes j and j + 64.
+// Edge-balanced: wavefront w takes the CSR edges [nnz*w/NW, nnz*(w+1)/NW)
+// (nnz = rowptr[M]), walks the rows they belong to, and for each row segment
+// folds the segment's columns into lane j's minimum of hash j (and j + 64);
+// columns are broadcast 64 at a time with readlane.  A segment that is a
+// whole row is stored, a partial one (a row cut by a range boundary, such as
+// a hub row spread over many wavefronts) is merged with atomicMin into the
+// preset 0xFFFFFFFF.  Empty rows keep the preset.
 __global__ __launch_bounds__(256) void k_minhash(const int32_t *rowptr, const int32_t *col,
                                                 int32_t M, int K, uint32_t seed,
                                                 uint32_t *sig) {
   const int lane = lane_id();
-  const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (row >= M) return;
+  const int64_t nw = (int64_t)gridDim.x * 4;
+  const int64_t w = (int64_t)blockIdx.x * 4 + wave_uniform(threadIdx.x >> 6);
+  const int64_t nnz = wave_uniform(rowptr[M]);
+  const int e_begin = (int)(nnz * w / nw);
+  const int e_end = (int)(nnz * (w + 1) / nw);
+  if (e_begin >= e_end) return;
   const uint32_t s0 = fmix32((uint32_t)lane * 0x9E3779B9u + seed);
   const uint32_t s1 = fmix32((uint32_t)(lane + kWave) * 0x9E3779B9u + seed);
-  uint32_t m0 = 0xFFFFFFFFu, m1 = 0xFFFFFFFFu;
-  const int b = wave_uniform(rowptr[row]);
-  const int e = wave_uniform(rowptr[row + 1]);
   const bool two = K > kWave;
-  for (int p = b; p < e; p += kWave) {
-    const int cnt = min(kWave, e - p);
-    const int c = lane < cnt ? col[p + lane] : 0;
-    for (int t = 0; t < cnt; ++t) {
-      const uint32_t x = (uint32_t)__builtin_amdgcn_readlane(c, t);
-      m0 = min(m0, fmix32(x ^ s0));
-      if (two) m1 = min(m1, fmix32(x ^ s1));
+  // the row holding e_begin: the last r with rowptr[r] <= e_begin
+  int r = wave_uniform(wave_lower_bound<int32_t>(rowptr, 0, M + 1, e_begin + 1) - 1);
+  int p = e_begin;
+  while (p < e_end) {
+    const int rs = wave_uniform(rowptr[r]);
+    const int re = wave_uniform(rowptr[r + 1]);
+    const int q = min(re, e_end);
+    uint32_t m0 = 0xFFFFFFFFu, m1 = 0xFFFFFFFFu;
+    for (int c0 = p; c0 < q; c0 += kWave) {
+      const int cnt = min(kWave, q - c0);
+      const int c = lane < cnt ? col[c0 + lane] : 0;
+      for (int t = 0; t < cnt; ++t) {
+        const uint32_t x = (uint32_t)__builtin_amdgcn_readlane(c, t);
+        m0 = min(m0, fmix32(x ^ s0));
+        if (two) m1 = min(m1, fmix32(x ^ s1));
+      }
     }
+    uint32_t *out = sig + (int64_t)r * K;
+    if (p == rs && q == re) {
+      if (lane < K) out[lane] = m0;
+      if (lane + kWave < K) out[lane + kWave] = m1;
+    } else {
+      if (lane < K) atomicMin(&out[lane], m0);
+      if (lane + kWave < K) atomicMin(&out[lane + kWave], m1);
+    }
+    p = q;
+    ++r;
+    while (r < M && wave_uniform(rowptr[r + 1]) <= p) ++r;  // skip empty rows
   }
-  uint32_t *out = sig + row * K;
-  if (lane < K) out[lane] = m0;
-  if (lane + kWave < K) out[lane + kWave] = m1;
 }
 
 __global__ void k_band_keys(const uint32_t *sig, int32_t n, int K, int band, int r,
@@ -154,37 +189,57 @@ __global__ void k_band_keys(const uint32_t *sig, int32_t n, int K, int band, int
 
 // Candidate pairs of one band: sorted positions i and i + d (d <= D) with
 // equal keys, both rows non-empty.  Pairs are packed (min << pb) | max and
-// appended at count[0]; writes past `cap` are dropped (the count still grows,
-// so the host sees the overflow).
+// appended at count[0] with one atomic per workgroup (the workgroup's pairs
+// are counted from the per-d ballots first); writes past `cap` are dropped
+// (the count still grows, so the host sees the overflow).
+constexpr int kMaxNeighbors = 16;
+
 __global__ __launch_bounds__(256) void k_emit_pairs(const uint64_t *keys, const int32_t *rows,
                                                    int32_t n, const int32_t *set_ptr, int D,
                                                    int pb, uint64_t *pairs, int64_t cap,
                                                    unsigned long long *count) {
+  __shared__ unsigned s_cnt[4];
+  __shared__ unsigned long long s_base;
   const int lane = lane_id();
+  const int wave = threadIdx.x >> 6;
   const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool in = i < n;
   const uint64_t ki = in ? keys[i] : 0ull;
   const int u = in ? rows[i] : 0;
   const bool u_live = in && set_ptr[u + 1] > set_ptr[u];
-  for (int d = 1; d <= D; ++d) {
+  uint64_t pk[kMaxNeighbors];
+  uint64_t mask[kMaxNeighbors];
+  unsigned total = 0;
+#pragma unroll
+  for (int d = 1; d <= kMaxNeighbors; ++d) {
     bool emit = false;
-    uint64_t pk = 0;
-    if (u_live && i + d < n && keys[i + d] == ki) {
+    pk[d - 1] = 0;
+    if (d <= D && u_live && i + d < n && keys[i + d] == ki) {
       const int v = rows[i + d];
       if (set_ptr[v + 1] > set_ptr[v]) {
         emit = true;
-        pk = ((uint64_t)min(u, v) << pb) | (uint64_t)max(u, v);
+        pk[d - 1] = ((uint64_t)min(u, v) << pb) | (uint64_t)max(u, v);
       }
     }
-    const uint64_t mask = __ballot(emit);
-    if (mask == 0) continue;
-    const int leader = __ffsll((long long)mask) - 1;
-    unsigned long long base = 0;
-    if (lane == leader) base = atomicAdd(count, (unsigned long long)__popcll(mask));
-    base = __shfl(base, leader, kWave);
-    const int64_t pos = (int64_t)base + __popcll(mask & lt_mask);
-    if (emit && pos < cap) pairs[pos] = pk;
+    mask[d - 1] = __ballot(emit);
+    total += (unsigned)__popcll(mask[d - 1]);
+  }
+  if (lane == 0) s_cnt[wave] = total;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned all = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+    s_base = all ? atomicAdd(count, (unsigned long long)all) : 0ull;
+  }
+  __syncthreads();
+  int64_t pos = (int64_t)s_base;
+  for (int w2 = 0; w2 < wave; ++w2) pos += s_cnt[w2];
+#pragma unroll
+  for (int d = 0; d < kMaxNeighbors; ++d) {
+    const uint64_t m = mask[d];
+    const int64_t at = pos + __popcll(m & lt_mask);
+    if (((m >> lane) & 1ull) && at < cap) pairs[at] = pk[d];
+    pos += __popcll(m);
   }
 }
 
@@ -201,12 +256,19 @@ __global__ void k_unique_pairs(const uint64_t *keys, const int32_t *G, int64_t n
   }
 }
 
-// One wavefront per pair.
+// One wavefront per pair.  The longer list, when it has at most
+// kJaccardLds entries, is staged in the wavefront's LDS slice and the shorter
+// list's entries (one per lane, 64 at a time) are binary-searched there;
+// longer lists are searched in global memory.
+constexpr int kJaccardLds = 512;
+
 __global__ __launch_bounds__(256) void k_jaccard(const int32_t *set_ptr, const int32_t *col,
                                                 const uint64_t *pairs, int64_t P,
                                                 float *sims) {
+  __shared__ int32_t s_b[4][kJaccardLds];
   const int lane = lane_id();
-  const int64_t p = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int wave = wave_uniform(threadIdx.x >> 6);
+  const int64_t p = (int64_t)blockIdx.x * 4 + wave;
   if (p >= P) return;
   const uint64_t pk = pairs[p];
   int u = (int)(pk >> 32), v = (int)(pk & 0xFFFFFFFFu);
@@ -219,16 +281,34 @@ __global__ __launch_bounds__(256) void k_jaccard(const int32_t *set_ptr, const i
     va = t0;
     lb = t1;
   }
+  la = wave_uniform(la);
+  lb = wave_uniform(lb);
   int hits = 0;
-  for (int k = lane; k < la; k += kWave) {
-    const int x = col[ua + k];
-    int lo = 0, hi = lb;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (col[va + mid] < x) lo = mid + 1;
-      else hi = mid;
+  if (lb <= kJaccardLds) {
+    int32_t *sb = s_b[wave];
+    for (int k = lane; k < lb; k += kWave) sb[k] = col[va + k];
+    wave_lds_sync();
+    for (int k = lane; k < la; k += kWave) {
+      const int x = col[ua + k];
+      int lo = 0, hi = lb;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (sb[mid] < x) lo = mid + 1;
+        else hi = mid;
+      }
+      hits += (lo < lb && sb[lo] == x) ? 1 : 0;
     }
-    hits += (lo < lb && col[va + lo] == x) ? 1 : 0;
+  } else {
+    for (int k = lane; k < la; k += kWave) {
+      const int x = col[ua + k];
+      int lo = 0, hi = lb;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (col[va + mid] < x) lo = mid + 1;
+        else hi = mid;
+      }
+      hits += (lo < lb && col[va + lo] == x) ? 1 : 0;
+    }
   }
   const int inter = wave_sum(hits);
   if (lane == 0)
@@ -263,15 +343,23 @@ __global__ __launch_bounds__(256) void k_union_keys(const int32_t *rowptr, const
   }
 }
 
-__global__ void k_union_fill(const uint64_t *keys, const int32_t *G, int64_t n, int cb,
-                             int32_t *ucol, int32_t *ucount) {
+// Unique (group, column) keys -> column list and group of every entry.
+__global__ void k_union_compact(const uint64_t *keys, const int32_t *G, int64_t n, int cb,
+                                int32_t *ucol, int32_t *ugroup) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint64_t k = keys[i];
   if (i == 0 || keys[i - 1] != k) {
     ucol[G[i] - 1] = (int32_t)(k & ((1ull << cb) - 1));
-    atomicAdd(&ucount[k >> cb], 1);
+    ugroup[G[i] - 1] = (int32_t)(k >> cb);
   }
+}
+
+// uptr[g] = first entry of group g (ugroup is non-decreasing), g = 0..C.
+__global__ void k_union_ptr(const int32_t *ugroup, int32_t U, int32_t C, int32_t *uptr) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g > C) return;
+  uptr[g] = lower_bound_dev<int32_t>(ugroup, 0, U, (int32_t)g);
 }
 
 // ---------------------------------------------------------------- host
@@ -313,7 +401,7 @@ dtc_status_t check_lsh_args(int32_t n, int32_t K, int32_t bands, int32_t r, int3
     return fail(DTC_ERROR_INVALID_VALUE, "num_hashes must be in [1, 128]");
   if (bands < 1 || r < 1 || (int64_t)bands * r > K)
     return fail(DTC_ERROR_INVALID_VALUE, "bands * rows_per_band must be in [1, num_hashes]");
-  if (D < 1) return fail(DTC_ERROR_INVALID_VALUE, "max_neighbors must be >= 1");
+  if (D < 1 || D > kMaxNeighbors) return fail(DTC_ERROR_INVALID_VALUE, "max_neighbors must be in [1, 16]");
   if (max_pairs < 0 || max_pairs > INT32_MAX)
     return fail(DTC_ERROR_INVALID_VALUE, "max_pairs must be in [0, 2^31)");
   return DTC_SUCCESS;
@@ -366,60 +454,39 @@ dtc_status_t lsh_pairs(const uint32_t *sig, const int32_t *set_ptr, int32_t n, i
   return DTC_SUCCESS;
 }
 
-// --- greedy merge (host), with the binary-heap discipline of a Python
-// queue.PriorityQueue of entries ordered by decreasing similarity, so that
-// equal similarities pop in the same order as the reference's queue.
+// Stream order of the pairs: a stable sort of the key-ordered pairs by
+// decreasing similarity (similarities are >= 0, so ~bits orders them).
+__global__ void k_sim_keys(const float *sims, int64_t P, uint64_t *keys, int32_t *vals) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  keys[i] = (uint64_t)(0xFFFFFFFFu - __float_as_uint(sims[i]));
+  vals[i] = (int32_t)i;
+}
+
+// --- greedy merge (host).  The queue pops the most similar pair first and,
+// among equal similarities, the smaller (p1, p2) first: a total order, so the
+// result does not depend on how the queue is built.  The initial pairs are
+// consumed as one stream pre-sorted in that order (sorted on the GPU by the
+// composite entry point); only re-queued pairs go through a binary heap.
+// "Is this pair queued" covers both: an initial pair is queued until the
+// stream passes it (its stream position is found by binary search of the
+// pairs in key order), a re-queued one while it sits in the heap.
 
 struct QItem {
-  int32_t p1, p2;
   double simi;
+  int32_t p1, p2;
 };
 
-inline bool q_less(const QItem &a, const QItem &b) { return a.simi > b.simi; }
-
-void sift_down(std::vector<QItem> &h, size_t start, size_t pos) {
-  const QItem item = h[pos];
-  while (pos > start) {
-    const size_t parent = (pos - 1) >> 1;
-    if (q_less(item, h[parent])) {
-      h[pos] = h[parent];
-      pos = parent;
-      continue;
-    }
-    break;
-  }
-  h[pos] = item;
+// a before b in pop order
+inline bool q_first(const QItem &a, const QItem &b) {
+  if (a.simi != b.simi) return a.simi > b.simi;
+  if (a.p1 != b.p1) return a.p1 < b.p1;
+  return a.p2 < b.p2;
 }
 
-void sift_up(std::vector<QItem> &h, size_t pos) {
-  const size_t end = h.size(), start = pos;
-  const QItem item = h[pos];
-  size_t child = 2 * pos + 1;
-  while (child < end) {
-    const size_t right = child + 1;
-    if (right < end && !q_less(h[child], h[right])) child = right;
-    h[pos] = h[child];
-    pos = child;
-    child = 2 * pos + 1;
-  }
-  h[pos] = item;
-  sift_down(h, start, pos);
-}
-
-void q_push(std::vector<QItem> &h, QItem x) {
-  h.push_back(x);
-  sift_down(h, 0, h.size() - 1);
-}
-
-QItem q_pop(std::vector<QItem> &h) {
-  const QItem last = h.back();
-  h.pop_back();
-  if (h.empty()) return last;
-  const QItem top = h[0];
-  h[0] = last;
-  sift_up(h, 0);
-  return top;
-}
+struct QLater {  // std::priority_queue keeps the element no other one precedes on top
+  bool operator()(const QItem &a, const QItem &b) const { return q_first(b, a); }
+};
 
 // |S1 ∩ S2| / |S1 ∪ S2| of sorted duplicate-free lists (TCA_reorder.py:95-100).
 double jaccard_host(const int32_t *a, int la, const int32_t *b, int lb) {
@@ -437,20 +504,139 @@ double jaccard_host(const int32_t *a, int la, const int32_t *b, int lb) {
   return (double)inter / (double)(la + lb - inter);
 }
 
-void greedy(int32_t n, const uint64_t *pairs, const float *sims, int64_t P,
-            const int32_t *lptr, const int32_t *lidx, int32_t thres, int32_t *root_out) {
-  auto makenum = [n](int64_t a, int64_t b) -> uint64_t {
-    return (uint64_t)(std::min(a, b) * (int64_t)n + std::max(a, b));
-  };
-  std::vector<QItem> que;
-  que.reserve((size_t)P);
-  std::unordered_set<uint64_t> sset;
-  sset.reserve((size_t)(P * 2 + 16));
+inline uint64_t pair_key(int32_t a, int32_t b) {
+  return ((uint64_t)(uint32_t)std::min(a, b) << 32) | (uint32_t)std::max(a, b);
+}
+
+// Open-addressing hash map from pair keys (< 2^63) to int64, linear probing,
+// tombstones on erase: one or two cache lines per lookup where a sorted-array
+// search of ten million keys costs twenty-odd misses.
+class PairMap {
+ public:
+  explicit PairMap(size_t expected) {
+    size_t cap = 16;
+    while (cap < 2 * expected + 16) cap <<= 1;
+    keys_.assign(cap, kEmpty);
+    vals_.assign(cap, 0);
+    mask_ = cap - 1;
+  }
+  void put(uint64_t k, int64_t v) {
+    if (2 * (used_ + 1) > keys_.size()) grow();
+    size_t i = slot(k), tomb = SIZE_MAX;
+    while (keys_[i] != kEmpty) {
+      if (keys_[i] == k) {
+        vals_[i] = v;
+        return;
+      }
+      if (keys_[i] == kTomb && tomb == SIZE_MAX) tomb = i;
+      i = (i + 1) & mask_;
+    }
+    if (tomb != SIZE_MAX) i = tomb;
+    else ++used_;
+    keys_[i] = k;
+    vals_[i] = v;
+  }
+  const int64_t *get(uint64_t k) const {
+    for (size_t i = slot(k); keys_[i] != kEmpty; i = (i + 1) & mask_)
+      if (keys_[i] == k) return &vals_[i];
+    return nullptr;
+  }
+  void erase(uint64_t k) {
+    for (size_t i = slot(k); keys_[i] != kEmpty; i = (i + 1) & mask_)
+      if (keys_[i] == k) {
+        keys_[i] = kTomb;
+        return;
+      }
+  }
+
+ private:
+  static constexpr uint64_t kEmpty = ~0ull, kTomb = ~0ull - 1;
+  size_t slot(uint64_t k) const { return (size_t)fmix64(k) & mask_; }
+  void grow() {  // rehash without tombstones, at least 4x the live keys
+    std::vector<uint64_t> ok;
+    std::vector<int64_t> ov;
+    ok.swap(keys_);
+    ov.swap(vals_);
+    size_t live = 0;
+    for (uint64_t k : ok) live += (k != kEmpty && k != kTomb) ? 1 : 0;
+    size_t cap = 16;
+    while (cap < 4 * live + 16) cap <<= 1;
+    keys_.assign(cap, kEmpty);
+    vals_.assign(cap, 0);
+    mask_ = keys_.size() - 1;
+    used_ = 0;
+    for (size_t i = 0; i < ok.size(); ++i)
+      if (ok[i] != kEmpty && ok[i] != kTomb) put(ok[i], ov[i]);
+  }
+  std::vector<uint64_t> keys_;
+  std::vector<int64_t> vals_;
+  size_t mask_ = 0, used_ = 0;  // used_ counts live keys and tombstones
+};
+
+// The initial pairs in stream (pop) order, and each pair's stream position.
+struct GreedyInput {
+  std::vector<QItem> stream;
+  PairMap rank{0};
+};
+
+// From key-sorted unique pairs (u < v), their similarities and the stream
+// order perm (stream position j holds key index perm[j]).
+GreedyInput greedy_input_sorted(const uint64_t *keys, const float *sims, const int32_t *perm,
+                                int64_t P) {
+  GreedyInput g;
+  g.stream.resize((size_t)P);
+  g.rank = PairMap((size_t)P);
+  for (int64_t j = 0; j < P; ++j) {
+    const int64_t k = perm[j];
+    g.stream[j] = QItem{(double)sims[k], (int32_t)(keys[k] >> 32), (int32_t)(keys[k] & 0xFFFFFFFFu)};
+    g.rank.put(keys[k], j);
+  }
+  return g;
+}
+
+// From pairs in any order (host sorts).  Pairs with a == b are dropped, and
+// of repeated pairs the first one given is kept (the reference's queue skips
+// later copies, TCA_reorder.py:139).
+GreedyInput greedy_input(const uint64_t *pairs, const float *sims, int64_t P) {
+  std::vector<int64_t> idx;
+  idx.reserve((size_t)P);
   for (int64_t i = 0; i < P; ++i) {
     const int32_t a = (int32_t)(pairs[i] >> 32), b = (int32_t)(pairs[i] & 0xFFFFFFFFu);
-    if (a == b || !sset.insert(makenum(a, b)).second) continue;
-    q_push(que, QItem{a, b, (double)sims[i]});
+    if (a != b) idx.push_back(i);
   }
+  auto key_of = [&](int64_t i) {
+    return pair_key((int32_t)(pairs[i] >> 32), (int32_t)(pairs[i] & 0xFFFFFFFFu));
+  };
+  std::sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) {
+    const uint64_t ka = key_of(a), kb = key_of(b);
+    return ka != kb ? ka < kb : a < b;
+  });
+  std::vector<uint64_t> keys;
+  std::vector<float> ksims;
+  for (size_t t = 0; t < idx.size(); ++t) {
+    const uint64_t k = key_of(idx[t]);
+    if (!keys.empty() && keys.back() == k) continue;
+    keys.push_back(k);
+    ksims.push_back(sims[idx[t]]);
+  }
+  const int64_t U = (int64_t)keys.size();
+  std::vector<int32_t> perm((size_t)U);
+  for (int64_t k = 0; k < U; ++k) perm[k] = (int32_t)k;
+  std::stable_sort(perm.begin(), perm.end(), [&](int32_t a, int32_t b) { return ksims[a] > ksims[b]; });
+  return greedy_input_sorted(keys.data(), ksims.data(), perm.data(), U);
+}
+
+void greedy(int32_t n, const GreedyInput &in, const int32_t *lptr, const int32_t *lidx,
+            int32_t thres, int32_t *root_out) {
+  const int64_t P = (int64_t)in.stream.size();
+  int64_t next = 0;  // stream position of the next initial pair
+  std::priority_queue<QItem, std::vector<QItem>, QLater> heap;
+  PairMap in_heap(1024);
+  auto queued = [&](uint64_t key) {
+    if (in_heap.get(key)) return true;
+    const int64_t *r = in.rank.get(key);
+    return r && *r >= next;
+  };
   std::vector<int32_t> cid(n), csz(n, 1);
   std::vector<uint8_t> deleted(n, 0);
   for (int32_t i = 0; i < n; ++i) cid[i] = i;
@@ -462,10 +648,16 @@ void greedy(int32_t n, const uint64_t *pairs, const float *sims, int64_t P,
     }
     return i;
   };
-  while (!que.empty() && num_cluster > 0) {
-    const QItem it = q_pop(que);
+  while ((next < P || !heap.empty()) && num_cluster > 0) {
+    QItem it;
+    if (heap.empty() || (next < P && q_first(in.stream[next], heap.top()))) {
+      it = in.stream[next++];
+    } else {
+      it = heap.top();
+      heap.pop();
+      in_heap.erase(pair_key(it.p1, it.p2));
+    }
     int32_t p1 = it.p1, p2 = it.p2;
-    sset.erase(makenum(p1, p2));
     if (p1 == cid[p1] && p2 == cid[p2]) {
       if (deleted[p1] || deleted[p2]) continue;
       if (csz[p1] < csz[p2]) std::swap(p1, p2);  // p1 absorbs p2 (ties: p1 keeps its root)
@@ -480,13 +672,34 @@ void greedy(int32_t n, const uint64_t *pairs, const float *sims, int64_t P,
       p1 = root(p1);
       p2 = root(p2);
       if (deleted[p1] || deleted[p2]) continue;
-      if (p1 != p2 && sset.insert(makenum(p1, p2)).second)
-        q_push(que, QItem{p1, p2,
-                          jaccard_host(lidx + lptr[p1], lptr[p1 + 1] - lptr[p1], lidx + lptr[p2],
-                                       lptr[p2 + 1] - lptr[p2])});
+      const uint64_t key = pair_key(p1, p2);
+      if (p1 != p2 && !queued(key)) {
+        heap.push(QItem{jaccard_host(lidx + lptr[p1], lptr[p1 + 1] - lptr[p1], lidx + lptr[p2],
+                                     lptr[p2 + 1] - lptr[p2]),
+                        p1, p2});
+        in_heap.put(key, 1);
+      }
     }
   }
   for (int32_t i = 0; i < n; ++i) root_out[i] = root(i);
+}
+
+// Stream order of P key-ordered pairs whose similarities are on the device
+// (sorted there; the LSH workspace's pair buffers are free again by now).
+dtc_status_t stream_order(const float *sims, int64_t P, const LshWs &w, hipStream_t stream,
+                          std::vector<int32_t> &perm) {
+  perm.resize((size_t)P);
+  if (P == 0) return DTC_SUCCESS;
+  k_sim_keys<<<blocks_for(P, 256), 256, 0, stream>>>(sims, P, w.pkeys0, w.pvals0);
+  DTC_LAUNCH_CHECK("sim_keys");
+  bool in_1 = false;
+  dtc_status_t st = radix_sort_pairs(w.pkeys0, w.pvals0, w.pkeys1, w.pvals1, P, 32, &in_1, w.tmp,
+                                     stream);
+  if (st != DTC_SUCCESS) return st;
+  DTC_HIP_CHECK(hipMemcpyAsync(perm.data(), in_1 ? w.pvals1 : w.pvals0, (size_t)P * 4,
+                               hipMemcpyDeviceToHost, stream));
+  DTC_HIP_CHECK(hipStreamSynchronize(stream));
+  return DTC_SUCCESS;
 }
 
 // Items grouped by root, groups in first-occurrence order, members
@@ -516,7 +729,7 @@ struct TcaWs {
   uint32_t *sig, *sig_c;
   uint64_t *pairs;
   float *sims;
-  int32_t *group, *ucount, *ucol;
+  int32_t *group, *ucount, *ucol, *ugroup;
   uint64_t *ukeys0, *ukeys1;
   int32_t *uvals0, *uvals1, *uG;
   void *utmp;
@@ -536,6 +749,7 @@ TcaWs carve_tca(void *base, int64_t M, int64_t nnz, int K, int64_t cap) {
   w.group = c.take<int32_t>(M);
   w.ucount = c.take<int32_t>(M + 1);
   w.ucol = c.take<int32_t>(nnz);
+  w.ugroup = c.take<int32_t>(nnz);
   w.ukeys0 = c.take<uint64_t>(nnz);
   w.ukeys1 = c.take<uint64_t>(nnz);
   w.uvals0 = c.take<int32_t>(nnz);
@@ -574,9 +788,12 @@ dtc_status_t dtc_minhash_signatures(const int32_t *node_pointer, const int32_t *
   if (num_rows == 0) return DTC_SUCCESS;
   if (!node_pointer || !edge_list || !signatures)
     return fail(DTC_ERROR_INVALID_VALUE, "node_pointer, edge_list and signatures are required");
-  k_minhash<<<(unsigned)ceil_div(num_rows, 4), 256, 0, stream>>>(node_pointer, edge_list,
-                                                                 num_rows, num_hashes, seed,
-                                                                 signatures);
+  const int64_t cells = (int64_t)num_rows * num_hashes;
+  k_fill_u32<<<blocks_for(cells, 256), 256, 0, stream>>>(signatures, cells, 0xFFFFFFFFu);
+  DTC_LAUNCH_CHECK("fill");
+  const unsigned grid = (unsigned)std::min<int64_t>(16384, ceil_div(num_rows, 4));
+  k_minhash<<<grid, 256, 0, stream>>>(node_pointer, edge_list, num_rows, num_hashes, seed,
+                                      signatures);
   DTC_LAUNCH_CHECK("minhash");
   return DTC_SUCCESS;
 }
@@ -640,7 +857,7 @@ dtc_status_t dtc_tca_greedy(int32_t num_items, const uint64_t *pairs, const floa
     if (a >= (uint64_t)num_items || b >= (uint64_t)num_items)
       return fail(DTC_ERROR_INVALID_VALUE, "pair " + std::to_string(i) + " names an item out of range");
   }
-  greedy(num_items, pairs, similarity, num_pairs, list_ptr, list_idx, thres, root);
+  greedy(num_items, greedy_input(pairs, similarity, num_pairs), list_ptr, list_idx, thres, root);
   return DTC_SUCCESS;
 }
 
@@ -681,8 +898,8 @@ dtc_status_t dtc_tca_reorder(const int32_t *node_pointer, const int32_t *edge_li
   // 1. CSR check: sorted, duplicate-free rows.
   k_fill_i32<<<1, 64, 0, stream>>>(w.flags, 2, 0);
   DTC_LAUNCH_CHECK("fill");
-  k_check_sorted_csr<<<blocks_for(M, 4), 256, 0, stream>>>(node_pointer, edge_list, num_rows,
-                                                           nnz, w.flags);
+  k_check_sorted_csr<<<(unsigned)std::min<int64_t>(4096, ceil_div(M, 4)), 256, 0, stream>>>(
+      node_pointer, edge_list, num_rows, nnz, w.flags);
   DTC_LAUNCH_CHECK("check_sorted_csr");
   int32_t flags[2] = {0, 0};
   DTC_HIP_CHECK(hipMemcpyAsync(flags, w.flags, 8, hipMemcpyDeviceToHost, stream));
@@ -714,8 +931,11 @@ dtc_status_t dtc_tca_reorder(const int32_t *node_pointer, const int32_t *edge_li
 
   // 3. Row clusters (TCU-aware level).
   std::vector<int32_t> roots((size_t)M), group, gptr, members;
-  greedy(num_rows, h_pairs.data(), h_sims.data(), P, h_ptr.data(), h_col.data(), o.thres,
-         roots.data());
+  std::vector<int32_t> perm;
+  st = stream_order(w.sims, P, lw, stream, perm);
+  if (st != DTC_SUCCESS) return st;
+  greedy(num_rows, greedy_input_sorted(h_pairs.data(), h_sims.data(), perm.data(), P), h_ptr.data(),
+         h_col.data(), o.thres, roots.data());
   const int32_t C = group_in_order(roots, group, gptr, members);
 
   // 4. Cluster unions: signatures and column lists.
@@ -724,8 +944,6 @@ dtc_status_t dtc_tca_reorder(const int32_t *node_pointer, const int32_t *edge_li
   DTC_LAUNCH_CHECK("fill");
   k_group_min<<<blocks_for(M * K, 256), 256, 0, stream>>>(w.sig, num_rows, K, w.group, w.sig_c);
   DTC_LAUNCH_CHECK("group_min");
-  k_fill_i32<<<blocks_for((int64_t)C + 1, 256), 256, 0, stream>>>(w.ucount, (int64_t)C + 1, 0);
-  DTC_LAUNCH_CHECK("fill");
   int64_t U = 0;
   if (nnz > 0) {
     const int cb = bits_of(std::max(flags[1], 1));
@@ -741,16 +959,17 @@ dtc_status_t dtc_tca_reorder(const int32_t *node_pointer, const int32_t *edge_li
     const uint64_t *sorted = in_1 ? w.ukeys1 : w.ukeys0;
     st = scan_key_changes(sorted, w.uG, nnz, w.utmp, stream);
     if (st != DTC_SUCCESS) return st;
-    k_union_fill<<<blocks_for(nnz, 256), 256, 0, stream>>>(sorted, w.uG, nnz, cb, w.ucol,
-                                                           w.ucount + 1);
-    DTC_LAUNCH_CHECK("union_fill");
+    k_union_compact<<<blocks_for(nnz, 256), 256, 0, stream>>>(sorted, w.uG, nnz, cb, w.ucol,
+                                                              w.ugroup);
+    DTC_LAUNCH_CHECK("union_compact");
     int32_t u32 = 0;
     DTC_HIP_CHECK(hipMemcpyAsync(&u32, w.uG + (nnz - 1), 4, hipMemcpyDeviceToHost, stream));
     DTC_HIP_CHECK(hipStreamSynchronize(stream));
     U = u32;
   }
-  st = scan_i32(w.ucount, w.ucount, (int64_t)C + 1, true, w.utmp, stream);
-  if (st != DTC_SUCCESS) return st;
+  k_union_ptr<<<blocks_for((int64_t)C + 1, 256), 256, 0, stream>>>(w.ugroup, (int32_t)U, C,
+                                                                   w.ucount);
+  DTC_LAUNCH_CHECK("union_ptr");
 
   // 5. Cluster pairs and the cache-aware level.
   int64_t Pc = 0;
@@ -771,8 +990,10 @@ dtc_status_t dtc_tca_reorder(const int32_t *node_pointer, const int32_t *edge_li
     DTC_HIP_CHECK(hipMemcpyAsync(c_col.data(), w.ucol, (size_t)U * 4, hipMemcpyDeviceToHost, stream));
   DTC_HIP_CHECK(hipStreamSynchronize(stream));
   std::vector<int32_t> c_roots((size_t)C), sgroup, sptr, smembers;
-  greedy(C, c_pairs.data(), c_sims.data(), Pc, c_ptr.data(), c_col.data(), o.cluster_thres,
-         c_roots.data());
+  st = stream_order(w.sims, Pc, lw, stream, perm);
+  if (st != DTC_SUCCESS) return st;
+  greedy(C, greedy_input_sorted(c_pairs.data(), c_sims.data(), perm.data(), Pc), c_ptr.data(),
+         c_col.data(), o.cluster_thres, c_roots.data());
   const int32_t S = group_in_order(c_roots, sgroup, sptr, smembers);
 
   // 6. Rows cluster by cluster, clusters super-cluster by super-cluster

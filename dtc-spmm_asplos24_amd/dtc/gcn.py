@@ -5,9 +5,9 @@ X W is a plain dense GEMM (torch / hipBLASLt, bf16 operands, float32
 accumulation, bf16 result); its bf16 output is gathered by the SpMM as is
 (b_dtype bfloat16: half the bytes of a float32 B), with A's values split into
 three bf16 limbs and float32 accumulation.  The backward pass:
-    dXW     = A^T dH            (ME-TCF of A^T, float32 SpMM)
+    dXW     = A^T dH            (ME-TCF of A^T; dH gathered in bf16 for a bf16 layer)
     dX      = dXW W^T,  dW = X^T dXW   (dense GEMMs; dW split over K)
-    dvalues = SDDMM(dH, XW)     on A's pattern.
+    dvalues = SDDMM(dH, XW)     on A's pattern, XW rows gathered as bf16.
 The reference's GCN entry (`DTCSpMM_gcn`, DTCSpMM_kernel.cu:1033-1116) is a
 forward only and its "backward" names return A * dY (DTCSpMM.cpp:384-399).
 """
@@ -52,11 +52,15 @@ class _GCNFunction(torch.autograd.Function):
         dH = dH.contiguous().float()
         d_values = d_X = d_W = None
         if ctx.needs_input_grad[0]:
-            d_values = ext.sddmm_csr(A.rowptr, A.col, dH, XW.float().contiguous(), A.edge_to_row)
+            # XW is gathered as stored (bf16 rows: half the bytes), exactly
+            d_values = ext.sddmm_csr(A.rowptr, A.col, dH, XW, A.edge_to_row)
         if ctx.needs_input_grad[1] or ctx.needs_input_grad[2]:
             At = A.transpose()
             vals_t_tc = ext.gather(ext.gather(values.contiguous(), A.perm_t), At.tc_edge)
-            dXW = At.spmm(dH, schedule=ctx.schedule, precision="fp32", values_tc=vals_t_tc)
+            # A^T dH feeds bf16 GEMMs: gather dH in X's precision (bf16 B
+            # kernel, float32 accumulation), then round the result once
+            dHx = dH.to(X.dtype) if X.dtype == torch.bfloat16 else dH
+            dXW = At.spmm(dHx, schedule=ctx.schedule, values_tc=vals_t_tc)
             dXW = dXW.to(X.dtype)
             if ctx.needs_input_grad[1]:
                 d_X = torch.matmul(dXW, W.t().contiguous())

@@ -33,6 +33,18 @@ def canonical_rows(rowptr, col, num_rows, values=None):
     return m.indptr.astype(np.int32), m.indices.astype(np.int32), m.data.astype(np.float32)
 
 
+def is_canonical(rowptr, col) -> bool:
+    """Columns strictly increasing inside every row (vectorised check)."""
+    col = np.asarray(col)
+    if col.shape[0] < 2:
+        return True
+    inc = np.diff(col.astype(np.int64)) > 0
+    cross = np.asarray(rowptr, np.int64)[1:-1] - 1  # diff index that enters a new row
+    cross = cross[(cross >= 0) & (cross < inc.shape[0])]
+    inc[cross] = True
+    return bool(inc.all())
+
+
 def tca_reorder(rowptr, col, num_rows=None, *, thres=16, cluster_thres=128, bands=28,
                 rows_per_band=2, max_neighbors=8, seed=2022, max_pairs=0, device="cuda"):
     """(reorder_id int64 [num_rows], stats dict)."""
@@ -40,7 +52,10 @@ def tca_reorder(rowptr, col, num_rows=None, *, thres=16, cluster_thres=128, band
     ext = load_extension()
     rowptr = np.asarray(rowptr)
     num_rows = rowptr.shape[0] - 1 if num_rows is None else int(num_rows)
-    rp, cc, _ = canonical_rows(rowptr, col, num_rows)
+    if is_canonical(rowptr, col):
+        rp, cc = np.asarray(rowptr, np.int32), np.asarray(col, np.int32)
+    else:
+        rp, cc, _ = canonical_rows(rowptr, col, num_rows)
     order, stats = ext.tca_reorder(torch.from_numpy(rp).to(device), torch.from_numpy(cc).to(device),
                                    thres, cluster_thres, bands, rows_per_band, max_neighbors, seed,
                                    max_pairs)

@@ -187,13 +187,14 @@ dtc_status_t dtc_csr_transpose(const int32_t *node_pointer, const int32_t *edge_
                                void *workspace, size_t workspace_bytes, dtc_stream_t stream);
 
 /* out[e] = sum_j x[row(e), j] * y[col(e), j] for every CSR edge e
- * (x: [num_rows x n], y: [y_rows x n]; columns >= y_rows give 0).
+ * (x: float32 [num_rows x n], y: [y_rows x n] float32 or bfloat16 per y_dtype
+ * (a dtc_b_dtype_t; bf16 taken exactly); columns >= y_rows give 0).
  * edge_to_row (the edgeToRow output of the ME-TCF build) may be NULL: rows
  * are then found by binary search of node_pointer. */
 dtc_status_t dtc_sddmm_csr(const int32_t *node_pointer, const int32_t *edge_list,
                            const int32_t *edge_to_row, int32_t num_rows, int32_t nnz,
-                           const float *x, const float *y, int32_t y_rows, int32_t n, float *out,
-                           dtc_stream_t stream);
+                           const float *x, const void *y, int32_t y_dtype, int32_t y_rows,
+                           int32_t n, float *out, dtc_stream_t stream);
 
 /* out[p] = in[idx[p]] for p < count (moves values between CSR and TC-block order). */
 dtc_status_t dtc_gather_f32(const float *in, const int32_t *idx, int32_t count, float *out,

@@ -121,17 +121,6 @@ def jaccard_pairs(rowptr, col, pairs):
     return out
 
 
-class _Pair:
-    """Queue entry ordered by decreasing similarity (TCA_reorder.py:102-110)."""
-    __slots__ = ("p1", "p2", "simi")
-
-    def __init__(self, p1, p2, simi):
-        self.p1, self.p2, self.simi = p1, p2, simi
-
-    def __lt__(self, other):
-        return self.simi > other.simi
-
-
 def _jd(l1, l2):
     # TCA_reorder.py:95-100
     if len(l1) == 0 or len(l2) == 0:
@@ -142,8 +131,12 @@ def _jd(l1, l2):
 
 def greedy_cluster(n, pairs, sims, lists, thres):
     """Root of every item after the greedy merge (TCA_reorder.py:152-187 for
-    rows, :250-286 for clusters).  `pairs` are inserted in the given order;
-    `lists[i]` are the column sets used when a stale pair is re-queued."""
+    rows, :250-286 for clusters).  The queue pops the most similar pair first
+    and, among equal similarities, the smaller (p1, p2) first (a total order;
+    the reference's PriorityQueue breaks such ties by its heap's internal
+    arrangement).  Of repeated pairs the first given is kept, pairs (a, a)
+    are skipped (:139); `lists[i]` are the column sets used when a stale pair
+    is re-queued with the Jaccard of the two roots' own lists (:182-187)."""
     que = []
     sset = set()
 
@@ -154,7 +147,7 @@ def greedy_cluster(n, pairs, sims, lists, thres):
         a, b = p >> 32, p & 0xFFFFFFFF
         if a == b or makenum(a, b) in sset:
             continue
-        heapq.heappush(que, _Pair(a, b, float(s)))
+        heapq.heappush(que, (-float(s), a, b))
         sset.add(makenum(a, b))
     cid = list(range(n))
     csz = [1] * n
@@ -168,8 +161,7 @@ def greedy_cluster(n, pairs, sims, lists, thres):
         return i
 
     while que and num_cluster > 0:
-        item = heapq.heappop(que)
-        p1, p2 = item.p1, item.p2
+        _, p1, p2 = heapq.heappop(que)
         sset.remove(makenum(p1, p2))
         if p1 == cid[p1] and p2 == cid[p2]:
             if deleted[p1] or deleted[p2]:
@@ -193,7 +185,7 @@ def greedy_cluster(n, pairs, sims, lists, thres):
             if deleted[p1] or deleted[p2]:
                 continue
             if p1 != p2 and makenum(p1, p2) not in sset:
-                heapq.heappush(que, _Pair(p1, p2, _jd(lists[p1], lists[p2])))
+                heapq.heappush(que, (-_jd(lists[p1], lists[p2]), p1, p2))
                 sset.add(makenum(p1, p2))
     return np.array([root(i) for i in range(n)], np.int64)
 

@@ -32,10 +32,11 @@ def test_transpose_vs_oracle(ext):
 
 @pytest.mark.parametrize("N", [1, 3, 16, 64, 130, 256, 300, 512])
 @pytest.mark.parametrize("with_rows", [False, True])
-def test_sddmm_vs_oracle(ext, N, with_rows):
+@pytest.mark.parametrize("y_bf16", [False, True])
+def test_sddmm_vs_oracle(ext, N, with_rows, y_bf16):
     """Edge-parallel SDDMM vs float64, rows from edgeToRow or from the row
-    pointer's binary search; a 3,000-entry hub row, empty rows, nnz not a
-    multiple of 8."""
+    pointer's binary search, Y float32 or bfloat16 (taken exactly); a
+    3,000-entry hub row, empty rows, nnz not a multiple of 8."""
     rng = np.random.default_rng(N)
     M, K = 700, 3500
     rowptr, col = random_csr(rng, M, K, 8.0, empty_frac=0.1)
@@ -47,11 +48,14 @@ def test_sddmm_vs_oracle(ext, N, with_rows):
     rowptr = np.concatenate([[0], np.cumsum([len(c) for c in cols])]).astype(np.int32)
     col = np.concatenate(cols).astype(np.int32)
     X = rng.standard_normal((M, N)).astype(np.float32)
-    Y = rng.standard_normal((K, N)).astype(np.float32)
-    say("sddmm", N, with_rows)
+    Yt = torch.from_numpy(rng.standard_normal((K, N)).astype(np.float32))
+    if y_bf16:
+        Yt = Yt.to(torch.bfloat16)
+    Y = Yt.float().numpy()
+    say("sddmm", N, with_rows, y_bf16)
     e2r = torch.from_numpy(np.repeat(np.arange(M), np.diff(rowptr)).astype(np.int32)).cuda()
     out = ext.sddmm_csr(torch.from_numpy(rowptr).cuda(), torch.from_numpy(col).cuda(),
-                        torch.from_numpy(X).cuda(), torch.from_numpy(Y).cuda(),
+                        torch.from_numpy(X).cuda(), Yt.cuda(),
                         e2r if with_rows else None).cpu().numpy()
     ref = oracle.sddmm_csr_f64(rowptr, col, X, Y, M)
     rows = np.repeat(np.arange(M), np.diff(rowptr))

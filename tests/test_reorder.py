@@ -26,12 +26,12 @@ def _pack(pairs):
     return np.array([(a << 32) | b for a, b in pairs], np.uint64)
 
 
-# Hand simulation of TCA_reorder.py:152-187 with Python's heapq discipline:
-# pairs (0,1) 1.0, (1,2) 2/3, (3,4) 2/3, (2,3) 0.0 pushed in that order,
-# thres 3.  Pops: (0,1) merges (equal sizes: 0 absorbs 1); of the tied pairs
-# (3,4) pops first (heap sift order) and merges; (1,2) is stale -> re-queued
-# as (0,2) with jd({1,2,3},{1,2}) = 2/3; (0,2) merges and freezes cluster 0
-# at size 3; (2,3) is stale and its root 0 is frozen -> dropped.
+# Hand simulation of TCA_reorder.py:152-187 with ties popped in (p1, p2)
+# order: pairs (0,1) 1.0, (1,2) 2/3, (3,4) 2/3, (2,3) 0.0, thres 3.  Pops:
+# (0,1) merges (equal sizes: 0 absorbs 1); of the tied pairs (1,2) comes
+# first and is stale -> re-queued as (0,2) with jd({1,2,3},{1,2}) = 2/3;
+# (0,2) ties with (3,4) and comes first: merges and freezes cluster 0 at
+# size 3; (3,4) merges; (2,3) is stale and its root 0 is frozen -> dropped.
 GREEDY_LISTS = [[1, 2, 3], [1, 2, 3], [1, 2], [7, 8], [7, 8, 9], []]
 GREEDY_PAIRS = [(0, 1, 1.0), (1, 2, 2 / 3), (3, 4, 2 / 3), (2, 3, 0.0)]
 GREEDY_ROOTS = [0, 0, 0, 3, 3, 5]
@@ -75,8 +75,9 @@ def test_host_greedy_hand_fixture(ext):
 @pytest.mark.parametrize("seed", range(6))
 @pytest.mark.parametrize("thres", [2, 3, 16])
 def test_host_greedy_matches_oracle(ext, seed, thres):
-    """libdtc's C++ heap against CPython's heapq, with many tied
-    similarities (quantised to quarters) so that the sift order matters."""
+    """libdtc's C++ greedy (pre-sorted stream + heap of re-queued pairs)
+    against the oracle's single heapq, with many tied similarities
+    (quantised to quarters) so that the tie order matters."""
     rng = np.random.default_rng(seed)
     n = int(rng.integers(20, 400))
     lists = [np.unique(rng.integers(0, 60, rng.integers(0, 12))).tolist() for _ in range(n)]
