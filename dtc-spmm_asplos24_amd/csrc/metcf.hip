@@ -122,30 +122,66 @@ __global__ __launch_bounds__(256) void k_validate(const int32_t *rowptr, const i
     flags |= __shfl_xor(flags, d, 64);
     mx = max(mx, __shfl_xor(mx, d, 64));
   }
+  // one atomic pair per workgroup
+  __shared__ int s_f[4], s_m[4];
   if ((threadIdx.x & 63) == 0) {
-    if (flags) atomicOr(&h->flags, flags);
-    atomicMax(&h->max_col, mx);
+    s_f[threadIdx.x >> 6] = flags;
+    s_m[threadIdx.x >> 6] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int bf = s_f[0] | s_f[1] | s_f[2] | s_f[3];
+    if (bf) atomicOr(&h->flags, bf);
+    atomicMax(&h->max_col, max(max(s_m[0], s_m[1]), max(s_m[2], s_m[3])));
   }
 }
 
-// edgeToRow (DTCSpMM_kernel.cu:107-120) and the sort keys, one wavefront per
-// row.
+// edgeToRow (DTCSpMM_kernel.cu:107-120) and the sort keys, edge-parallel:
+// wavefront w takes the edges [nnz*w/NW, nnz*(w+1)/NW) and walks them 64 at a
+// time.  Lane j holds rowptr[r + 1 + j], the end of row r + j, for the 64 rows
+// from the row r containing the next edge; every edge of a step lies before
+// the last of those ends, and its row is r + (the first j whose end exceeds
+// it), found by a 6-step binary search over the lanes.  A hub row is spread
+// over many wavefronts and short rows share one step, so no wavefront walks a
+// long row serially.
 __global__ __launch_bounds__(256) void k_rows_keys(const int32_t *rowptr, const int32_t *col,
-                                                   int32_t M, int32_t blk_h, int colbits,
-                                                   int32_t *e2r, uint64_t *keys,
+                                                   int32_t M, int32_t nnz, int32_t blk_h,
+                                                   int colbits, int32_t *e2r, uint64_t *keys,
                                                    int32_t *vals) {
-  const int lane = threadIdx.x & 63;
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  for (int64_t r = wave; r < M; r += nwaves) {
-    const int s = rowptr[r];
-    const int e = rowptr[r + 1];
-    const uint64_t hi = (uint64_t)(r / blk_h) << colbits;
-    for (int idx = s + lane; idx < e; idx += 64) {
-      e2r[idx] = (int32_t)r;
-      keys[idx] = hi | (uint32_t)col[idx];
+  const int lane = lane_id();
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+  const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave_uniform(threadIdx.x >> 6);
+  const int e_begin = (int)((int64_t)nnz * w / nw);
+  const int e_end = (int)((int64_t)nnz * (w + 1) / nw);
+  if (e_begin >= e_end) return;
+  // the row of e_begin: the last r with rowptr[r] <= e_begin
+  int r = wave_uniform(wave_lower_bound<int32_t>(rowptr, 0, M + 1, e_begin + 1) - 1);
+  int p = e_begin;
+  while (p < e_end) {
+    const int64_t bi = (int64_t)r + 1 + lane;
+    const int bnd = bi <= M ? rowptr[bi] : INT32_MAX;
+    const uint64_t past = __ballot(bnd > p);
+    if (past == 0) {  // rows r .. r+63 all end at or before p (empty rows)
+      r += kWave;
+      continue;
+    }
+    const int last = __builtin_amdgcn_readlane(bnd, kWave - 1);  // > p
+    const int cnt = min(min(kWave, e_end - p), last - p);
+    const int idx = p + min(lane, cnt - 1);
+    int lo = 0, hi = kWave - 1;  // first lane whose row end exceeds idx (lane 63's does)
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (__shfl(bnd, mid, kWave) > idx) hi = mid;
+      else lo = mid + 1;
+    }
+    const int row = r + lo;
+    if (lane < cnt) {
+      e2r[idx] = row;
+      keys[idx] = ((uint64_t)(row / blk_h) << colbits) | (uint32_t)col[idx];
       vals[idx] = idx;
     }
+    r = wave_uniform(__shfl(row, cnt - 1, kWave));  // row of the step's last edge
+    p += cnt;
   }
 }
 
@@ -375,8 +411,8 @@ dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *no
   bool in_1 = false;
   if (nnz > 0) {
     // 2. Keys (window, column) with the CSR edge index as value; stable sort.
-    const unsigned grid = (unsigned)std::min<int64_t>(65536, grid_for(num_nodes, 4));
-    k_rows_keys<<<grid, 256, 0, stream>>>(node_pointer, edge_list, num_nodes, blk_h, colbits,
+    const unsigned grid = (unsigned)std::min<int64_t>(65536, grid_for(nnz, 4096));
+    k_rows_keys<<<grid, 256, 0, stream>>>(node_pointer, edge_list, num_nodes, nnz, blk_h, colbits,
                                           edge_to_row, ws.keys0, ws.vals0);
     DTC_LAUNCH_CHECK("rows_keys");
     st = radix_sort_pairs(ws.keys0, ws.vals0, ws.keys1, ws.vals1, nnz, key_bits, &in_1, ws.tmp,
