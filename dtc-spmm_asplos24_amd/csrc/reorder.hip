@@ -4,15 +4,18 @@
 //
 // GPU, data-parallel over rows, edges and pairs:
 //   * MinHash signatures: sig[r][j] = min over the row's columns c of
-//     fmix32(c ^ s_j), s_j = fmix32(j * 0x9E3779B9 + seed); one wavefront per
-//     row, lane j owns hash j (and j + 64), columns broadcast with readlane.
+//     fmix32(c ^ s_j), s_j = fmix32(j * 0x9E3779B9 + seed); each wavefront
+//     takes an equal range of edges, lane j owns hash j (and j + 64), columns
+//     are broadcast with readlane.
 //   * LSH: per band, a stable radix sort of rows by band key; rows with equal
-//     keys at most D positions apart become candidate pairs (wave-aggregated
-//     appends), then one sort + unique over all bands.
+//     keys at most D positions apart become candidate pairs (one atomic
+//     append per workgroup), then one sort + unique over all bands.
 //   * Exact Jaccard of each pair over sorted rows, one wavefront per pair
-//     (binary search of the shorter row's columns in the longer row).
+//     (binary search of the shorter row's columns in the longer row, staged
+//     in LDS when it is short).
 //   * Cluster unions for the second level: entrywise-min signatures and the
 //     union column lists by a sort + unique of (cluster, column) keys.
+//   * The pairs' pop order (a stable sort by decreasing similarity).
 // Host: the greedy merge itself, which pops the most similar pair first and
 // re-queues stale pairs (TCA_reorder.py:152-187, :250-286) -- a sequential
 // priority-queue algorithm, as in the reference, kept in C++ here.

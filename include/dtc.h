@@ -275,9 +275,10 @@ dtc_status_t dtc_jaccard_pairs(const int32_t *set_ptr, const int32_t *set_idx,
                                const uint64_t *pairs, int64_t num_pairs, float *similarity,
                                dtc_stream_t stream);
 /* Host memory: the greedy merge of TCA_reorder.py:152-187 over num_items
- * items (pairs pushed in the given order, most similar popped first, stale
- * pairs re-queued with the Jaccard of the two roots' own lists); root[i] is
- * item i's cluster root. */
+ * items: most similar pair popped first (equal similarities in increasing
+ * (p1, p2) order), stale pairs re-queued with the Jaccard of the two roots'
+ * own lists (list_ptr/list_idx, sorted rows); pairs (a, a) are skipped and of
+ * repeated pairs the first given is kept.  root[i] is item i's cluster root. */
 dtc_status_t dtc_tca_greedy(int32_t num_items, const uint64_t *pairs, const float *similarity,
                             int64_t num_pairs, const int32_t *list_ptr, const int32_t *list_idx,
                             int32_t thres, int32_t *root);
