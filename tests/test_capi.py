@@ -47,9 +47,46 @@ def test_status_strings(lib):
     assert lib.dtc_version() >= 10000
 
 
-class Opts(ctypes.Structure):
+class Opts(ctypes.Structure):  # dtc_spmm_options_t
     _fields_ = [("precision", ctypes.c_int32), ("vec", ctypes.c_int32), ("subtiles", ctypes.c_int32),
-                ("chunk_blocks", ctypes.c_int32), ("persistent", ctypes.c_int32)]
+                ("chunk_blocks", ctypes.c_int32), ("persistent", ctypes.c_int32), ("b_dtype", ctypes.c_int32)]
+
+
+class TcaOpts(ctypes.Structure):  # dtc_tca_options_t
+    _fields_ = [("thres", ctypes.c_int32), ("cluster_thres", ctypes.c_int32), ("bands", ctypes.c_int32),
+                ("rows_per_band", ctypes.c_int32), ("max_neighbors", ctypes.c_int32), ("seed", ctypes.c_uint32),
+                ("max_pairs", ctypes.c_int64)]
+
+
+def test_struct_sizes_match_header():
+    assert ctypes.sizeof(Opts) == 24 and ctypes.sizeof(TcaOpts) == 32
+
+
+def test_reorder_and_sddmm_argument_errors(lib):
+    lib.dtc_last_error.restype = ctypes.c_char_p
+    size = ctypes.c_size_t(0)
+    good = TcaOpts(16, 128, 28, 2, 8, 2022, 1000)
+    assert lib.dtc_tca_reorder_workspace_size(1000, 5000, ctypes.byref(good), ctypes.byref(size)) == 0
+    assert size.value > 0
+    for bad in (TcaOpts(0, 128, 28, 2, 8, 1, 1000),      # thres < 1
+                TcaOpts(16, 128, 70, 2, 8, 1, 1000),     # 140 hashes > 128
+                TcaOpts(16, 128, 28, 2, 17, 1, 1000),    # max_neighbors > 16
+                TcaOpts(16, 128, 28, 2, 8, 1, -1)):      # negative capacity
+        assert lib.dtc_tca_reorder_workspace_size(1000, 5000, ctypes.byref(bad), ctypes.byref(size)) == 1
+    assert lib.dtc_tca_reorder_workspace_size(1000, 5000, None, ctypes.byref(size)) == 1
+    assert lib.dtc_lsh_pairs_workspace_size(-1, 10, ctypes.byref(size)) == 1
+    assert lib.dtc_minhash_signatures(None, None, 10, 0, 1, None, None) == 1  # zero hashes
+    # greedy: thres < 1, and a pair naming an item out of range (host memory only)
+    ptr = (ctypes.c_int32 * 3)(0, 0, 0)
+    root = (ctypes.c_int32 * 2)()
+    pair = (ctypes.c_uint64 * 1)((0 << 32) | 7)
+    sim = (ctypes.c_float * 1)(0.5)
+    assert lib.dtc_tca_greedy(2, pair, sim, 1, ptr, None, 0, root) == 1
+    assert lib.dtc_tca_greedy(2, pair, sim, 1, ptr, None, 16, root) == 1
+    assert b"out of range" in lib.dtc_last_error()
+    # sddmm: unknown y dtype is rejected before any launch
+    dummy = ctypes.c_void_p(16)
+    assert lib.dtc_sddmm_csr(dummy, dummy, None, 4, 4, dummy, dummy, 7, 4, 8, dummy, None) == 1
 
 
 def test_host_side_argument_errors(lib):
