@@ -145,6 +145,37 @@ def bench_gcn(args):
     }), flush=True)
 
 
+def gather_ceiling(A, B, chunk, stream, steps):
+    """Time of the gather-only probe (libdtc_probe.so): the SpMM's B-row loads
+    for this ME-TCF and chunk size, nothing else -- the practical ceiling of
+    the access pattern.  None when N is not a multiple of 64 or the probe
+    library is not built."""
+    import ctypes
+
+    import torch
+    path = os.path.join(PKG, "libdtc_probe.so")
+    N = int(B.shape[1])
+    if N % 64 or not os.path.exists(path):
+        return None
+    lib = ctypes.CDLL(path)
+    lib.dtc_gather_ceiling_out_size.restype = ctypes.c_int64
+    T = A.block_count
+    atox = A.sparse_atox
+    # every slot must index a row of B before the probe reads it
+    if T == 0 or int(atox.min()) < 0 or int(atox.max()) >= B.shape[0] or atox.numel() != 8 * T:
+        return None
+    out = torch.empty(lib.dtc_gather_ceiling_out_size(T, chunk, N), device=B.device)
+
+    def run():
+        rc = lib.dtc_gather_ceiling(ctypes.c_void_p(atox.data_ptr()), T, chunk, ctypes.c_void_p(B.data_ptr()), N,
+                                    ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(stream.cuda_stream))
+        if rc != 0:
+            raise RuntimeError(f"gather probe failed ({rc})")
+    run()
+    torch.cuda.synchronize()
+    return timed(stream, run, steps)
+
+
 def bench_reordered(a, B, C64, Cabs, mask, v, ms, flops, sum_u, W, N, stream, steps):
     """The chosen variant on the TCA-reordered graph (rows and columns
     relabelled as the reference's reordered datasets are): reorder time, the
@@ -429,6 +460,13 @@ def main():
                "host_metcf_build_ms": round(float(np.median(tb)) * 1e3, 1),
                "torch_sparse_csr_gflops": round(flops / t_ts / 1e9, 2)}
 
+    # ---- practical ceiling: the same B-row loads with nothing else
+    ceiling = None
+    if chosen_v[0] == "balanced" and chosen_v[3] == 4:
+        g_ms = gather_ceiling(A, B, chosen_v[2], stream, args.steps)
+        if g_ms:
+            ceiling = {"gather_only_ms": round(g_ms, 4), "spmm_frac_of_gather_ceiling": round(g_ms / ms, 4)}
+
     # ---- the same SpMM on the TCA-reordered graph (rows and columns relabelled)
     reordered = None
     if args.workload == "rmat" and not args.no_reorder:
@@ -464,6 +502,7 @@ def main():
         "error_vs_fp64": {"componentwise": comp_err, "normwise": norm_err, "target": ERR_TARGET},
         "fp32_path": fp32,
         "selector": selector_info,
+        "gather_ceiling": ceiling,
         "reordered": reordered,
         "preprocess_ms": round(float(np.median(pre_ms)), 3),
         "bytes_model_per_launch": bytes_model,

@@ -26,6 +26,9 @@ HIP_SOURCES = ["primitives.hip", "metcf.hip", "spmm.hip", "backward.hip", "selec
                "reorder.hip"]
 HEADERS = [os.path.join(CSRC, "dtc_internal.h"), os.path.join(INCLUDE, "dtc.h")]
 LIB = os.path.join(HERE, "libdtc.so")
+# measurement aid beside the library (not part of include/dtc.h)
+PROBE_SRC = "probe_gather.hip"
+PROBE_LIB = os.path.join(HERE, "libdtc_probe.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = os.environ.get("DTC_OFFLOAD_ARCH", "gfx950")
@@ -89,6 +92,16 @@ def build_lib(force: bool = False, verbose: bool = False, jobs: int = 4) -> str:
     return LIB
 
 
+def build_probe(force: bool = False, verbose: bool = False) -> str:
+    src = os.path.join(CSRC, PROBE_SRC)
+    if force or _stale(PROBE_LIB, [src]):
+        _, torch_lib, _ = _torch_paths()
+        _run([HIPCC, *HIP_FLAGS, "-shared", src, "-o", PROBE_LIB + ".tmp",
+              f"-Wl,-rpath,{torch_lib}:{ROCM}/lib"], verbose)
+        os.replace(PROBE_LIB + ".tmp", PROBE_LIB)
+    return PROBE_LIB
+
+
 def build_ext(force: bool = False, verbose: bool = False) -> str:
     out = ext_path()
     src = os.path.join(CSRC, "DTCSpMM_ext.cpp")
@@ -111,6 +124,7 @@ def build_ext(force: bool = False, verbose: bool = False) -> str:
 def build(force: bool = False, verbose: bool = False) -> None:
     build_lib(force=force, verbose=verbose)
     build_ext(force=force, verbose=verbose)
+    build_probe(force=force, verbose=verbose)
 
 
 if __name__ == "__main__":
@@ -122,4 +136,5 @@ if __name__ == "__main__":
     build_lib(force=a.force, verbose=a.verbose)
     if not a.lib_only:
         build_ext(force=a.force, verbose=a.verbose)
-    print("built", LIB, "" if a.lib_only else ext_path())
+        build_probe(force=a.force, verbose=a.verbose)
+    print("built", LIB, "" if a.lib_only else f"{ext_path()} {PROBE_LIB}")

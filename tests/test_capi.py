@@ -158,3 +158,17 @@ def test_extension_rejects_bad_plan_and_precision(ext):
         ext.run_DTCSpMM(X, rwo, tile, off, atox, 32, 4, "float_nonsplit", precision="tf32")
     with pytest.raises(ValueError, match="schedule"):
         ext.spmm(X, rwo, tile, off, atox, 32, schedule="diagonal")
+
+
+def test_gather_ceiling_probe_library():
+    """The measurement aid beside libdtc: exported entry points and host-side
+    size checks (no device work)."""
+    path = os.path.join(PKG, "libdtc_probe.so")
+    if not os.path.exists(path):
+        pytest.skip("libdtc_probe.so not built")
+    probe = ctypes.CDLL(path)
+    probe.dtc_gather_ceiling_out_size.restype = ctypes.c_int64
+    assert probe.dtc_gather_ceiling_out_size(1000, 256, 256) == 4 * 1 * 256
+    assert probe.dtc_gather_ceiling_out_size(1000, 256, 100) == -1  # N not a multiple of 64
+    assert probe.dtc_gather_ceiling_out_size(0, 256, 256) == -1
+    assert probe.dtc_gather_ceiling(None, 0, 256, None, 256, None, None) == 1
