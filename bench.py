@@ -460,6 +460,24 @@ def main():
                "host_metcf_build_ms": round(float(np.median(tb)) * 1e3, 1),
                "torch_sparse_csr_gflops": round(flops / t_ts / 1e9, 2)}
 
+    # ---- TC-GNN's SpMM on the same matrix (the comparison baseline without
+    # TC-block compaction; it rescans a window's edges for every group of blocks)
+    tcgnn = None
+    if not args.no_table:
+        from dtc import load_extension
+        ext = load_extension()
+
+        def run_tcgnn():
+            return ext.run_TCGNNSpMM(B, A.rowptr, A.col, A.block_partition, A.edge_to_column, A.edge_to_row,
+                                     values=A.values)[0]
+        Ct = run_tcgnn()
+        torch.cuda.synchronize()
+        t_err = errors(Ct)[0]
+        t_ms = timed(stream, run_tcgnn, 1)  # seconds per call on power-law graphs
+        tcgnn = {"ms": round(t_ms, 3), "gflops": round(flops / t_ms / 1e6, 2), "err": t_err,
+                 "chosen_speedup_over_tcgnn": round(t_ms / ms, 2)}
+        log(f"TC-GNN baseline: {t_ms:.3f} ms")
+
     # ---- practical ceiling: the same B-row loads with nothing else
     ceiling = None
     if chosen_v[0] == "balanced" and chosen_v[3] == 4:
@@ -503,6 +521,7 @@ def main():
         "fp32_path": fp32,
         "selector": selector_info,
         "gather_ceiling": ceiling,
+        "tcgnn_baseline": tcgnn,
         "reordered": reordered,
         "preprocess_ms": round(float(np.median(pre_ms)), 3),
         "bytes_model_per_launch": bytes_model,

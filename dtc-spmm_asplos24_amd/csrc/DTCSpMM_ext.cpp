@@ -253,6 +253,70 @@ py::tuple preprocess_gpu(Tensor edgeList, Tensor nodePointer, int64_t num_nodes,
   return py::tuple(out);
 }
 
+void check_csr_pair(const Tensor &ptr, const Tensor &idx);
+
+// TC-GNN's format (DTCSpMM.cpp:237-311): the reference builds it on the CPU
+// into caller-allocated CPU tensors.  Here the ME-TCF build's phase 1 computes
+// the same arrays on the GPU and they are copied back into the caller's
+// tensors.  One difference, deliberate: an empty row window gets 0 blocks
+// (the reference's CPU loop gives it 1, and writes blockPartition[W] out of
+// bounds when num_nodes % 16 == 0, DTCSpMM.cpp:281).
+int64_t preprocess(Tensor edgeList, Tensor nodePointer, int64_t num_nodes, int64_t blockSize_h,
+                   int64_t blockSize_w, Tensor blockPartition, Tensor edgeToColumn, Tensor edgeToRow) {
+  for (const Tensor *t : {&edgeList, &nodePointer, &blockPartition, &edgeToColumn, &edgeToRow}) {
+    CHECK_CONTIGUOUS(*t);
+    CHECK_DTYPE(*t, torch::kInt32);
+  }
+  TORCH_CHECK(torch::cuda::is_available(), "preprocess runs the ME-TCF build on the GPU; no GPU is visible");
+  const auto dev = torch::Device(torch::kCUDA);  // the current device
+  Tensor el = edgeList.to(dev), np = nodePointer.to(dev);
+  Tensor bp = torch::zeros({blockPartition.numel()}, el.options());
+  Tensor e2c = torch::empty({edgeToColumn.numel()}, el.options());
+  Tensor e2r = torch::empty({edgeToRow.numel()}, el.options());
+  py::tuple r = preprocess_gpu(el, np, num_nodes, blockSize_h, blockSize_w, bp, e2c, e2r, c10::nullopt, false);
+  blockPartition.copy_(bp);
+  edgeToColumn.copy_(e2c);
+  edgeToRow.copy_(e2r);
+  return r[5].cast<int64_t>();
+}
+
+std::vector<Tensor> run_TCGNNSpMM(Tensor input, Tensor nodePointer, Tensor edgeList, Tensor blockPartition,
+                                  Tensor edgeToColumn, Tensor edgeToRow, c10::optional<Tensor> values) {
+  CHECK_INPUT(input);
+  CHECK_DTYPE(input, torch::kFloat32);
+  check_csr_pair(nodePointer, edgeList);
+  for (const Tensor *t : {&blockPartition, &edgeToColumn, &edgeToRow}) {
+    CHECK_INPUT(*t);
+    CHECK_DTYPE(*t, torch::kInt32);
+    CHECK_SAME_DEVICE(*t, input);
+  }
+  CHECK_SAME_DEVICE(nodePointer, input);
+  TORCH_CHECK(input.dim() == 2, "input must be 2-D [rows, N]");
+  const int64_t M = nodePointer.numel() - 1, nnz = edgeList.numel();
+  TORCH_CHECK(blockPartition.numel() == (M + 15) / 16, "blockPartition must have one entry per row window");
+  TORCH_CHECK(edgeToColumn.numel() == nnz && edgeToRow.numel() == nnz, "edgeToColumn / edgeToRow need one entry per edge");
+  const float *vals = nullptr;
+  if (values.has_value()) {
+    CHECK_INPUT(*values);
+    CHECK_DTYPE(*values, torch::kFloat32);
+    TORCH_CHECK(values->numel() == nnz, "values must have one entry per edge");
+    vals = values->data_ptr<float>();
+  }
+  const c10::OptionalDeviceGuard guard(input.device());
+  // the reference returns zeros_like(input) filled by the kernel (DTCSpMM_kernel.cu:715)
+  Tensor out = M == input.size(0) ? torch::empty_like(input) : torch::zeros_like(input);
+  const int64_t m_eff = std::min<int64_t>(M, input.size(0));
+  maybe_bench("TCGNNSpMM", nnz, input.size(1), [&] {
+    check_status(dtc_spmm_tcgnn(nodePointer.data_ptr<int32_t>(), edgeList.data_ptr<int32_t>(),
+                                blockPartition.data_ptr<int32_t>(), edgeToColumn.data_ptr<int32_t>(),
+                                edgeToRow.data_ptr<int32_t>(), vals, (int32_t)m_eff, (int32_t)nnz,
+                                input.data_ptr<float>(), (int32_t)input.size(0), (int32_t)input.size(1),
+                                out.data_ptr<float>(), nullptr, current_stream()),
+                 "run_TCGNNSpMM");
+  });
+  return {out};
+}
+
 // --------------------------------------------------------------- SpMM
 
 std::vector<Tensor> run_windows(const Operands &o, const Tensor &rwo, int64_t num_nodes, int64_t num_edges,
@@ -668,6 +732,12 @@ py::tuple tca_reorder(Tensor nodePointer, Tensor edgeList, int64_t thres, int64_
 }  // namespace
 
 PYBIND11_MODULE(DTCSpMM, m) {
+  m.def("preprocess", &preprocess, "TC-GNN format (blockPartition, edgeToColumn, edgeToRow) into CPU tensors",
+        py::arg("edgeList"), py::arg("nodePointer"), py::arg("num_nodes"), py::arg("blockSize_h"),
+        py::arg("blockSize_w"), py::arg("blockPartition"), py::arg("edgeToColumn"), py::arg("edgeToRow"));
+  m.def("run_TCGNNSpMM", &run_TCGNNSpMM, "TC-GNN SpMM (comparison baseline)", py::arg("input"),
+        py::arg("nodePointer"), py::arg("edgeList"), py::arg("blockPartition"), py::arg("edgeToColumn"),
+        py::arg("edgeToRow"), py::arg("values") = py::none());
   m.def("minhash", &minhash, "MinHash signatures of CSR rows (uint32 bits in int32)", py::arg("nodePointer"),
         py::arg("edgeList"), py::arg("num_hashes") = 56, py::arg("seed") = 2022);
   m.def("lsh_pairs", &lsh_pairs, "LSH candidate pairs, packed u << 32 | v", py::arg("signatures"),

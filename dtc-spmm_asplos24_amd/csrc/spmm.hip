@@ -693,11 +693,135 @@ dtc_status_t zero_output(float *c, int32_t m, int32_t n, hipStream_t stream) {
 }
 
 }  // namespace
+// TC-GNN's SpMM (DTCSpMM.cpp:97-115, DTCSpMM_kernel.cu:703-808 and
+// :1566-1685), the comparison baseline without TC-block compaction.  Its
+// format is ME-TCF's phase 1 only: blockPartition, edgeToColumn, edgeToRow.
+// One unit per (row window, column tile); for every group of 4 blocks (32
+// condensed columns) the wave rescans ALL of the window's edges, scattering
+// those whose condensed column falls in the group into the LDS A tile and
+// recording each condensed column's original column (TC-GNN's per-block
+// sparse_AToX), then gathers B and runs the MFMAs as the ME-TCF kernels do.
+// The rescans -- window edges times groups -- are what ME-TCF's compaction
+// removes.
+template <int VEC>
+__global__ __launch_bounds__(256) void spmm_tcgnn_kernel(
+    const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+    const int32_t *__restrict__ e2c, const int32_t *__restrict__ e2r,
+    const int32_t *__restrict__ bpart, const float *__restrict__ vals, int32_t W,
+    SpmmParams p) {
+  __shared__ __attribute__((aligned(16))) float s_tile[kWavesPerBlock][kTileFloats];
+  __shared__ int32_t s_atox[kWavesPerBlock][kGroup * kBlkW];
+  const int wave = wave_uniform(threadIdx.x >> 6);
+  const int lane = lane_id();
+  const int ct = blockIdx.x % p.n_ct;
+  const int w = (blockIdx.x / p.n_ct) * kWavesPerBlock + wave;
+  if (w >= W) return;
+  const int colbase = ct * 16 * VEC + (lane & 15) * VEC;  // this lane's columns, as in run_unit
+  const int64_t r0 = (int64_t)w * kBlkH;
+  const int ws = wave_uniform(rowptr[min(r0, (int64_t)p.M)]);
+  const int we = wave_uniform(rowptr[min(r0 + kBlkH, (int64_t)p.M)]);
+  const int nb = wave_uniform(bpart[w]);
+  float *tile = s_tile[wave];
+  int32_t *atox = s_atox[wave];
+  floatx4 acc[1][VEC];
+  zero_acc<VEC, 1>(acc);
+  const int g = lane >> 4;
+  const int q = lane & 15;
+  for (int k0 = 0; k0 < nb; k0 += kGroup) {
+    floatx4 *t4 = reinterpret_cast<floatx4 *>(tile);
+    const floatx4 z4 = {0.f, 0.f, 0.f, 0.f};
+    t4[lane * 2] = z4;
+    t4[lane * 2 + 1] = z4;
+    if (lane < kGroup * kBlkW) atox[lane] = 0;  // padding slots gather row 0 times 0
+    wave_lds_sync();
+    for (int e = ws + lane; e < we; e += kWave) {
+      const int c = e2c[e];
+      const int gg = (c >> 3) - k0;
+      if (gg >= 0 && gg < kGroup) {
+        const int slot = c & (kBlkW - 1);
+        tile[gg * (kBlkH * kBlkW) + (e2r[e] & (kBlkH - 1)) * kBlkW + slot] = vals ? vals[e] : 1.0f;
+        atox[gg * kBlkW + slot] = col[e];
+      }
+    }
+    wave_lds_sync();
+    float a[8];
+    const floatx4 lo = t4[(g * (kBlkH * kBlkW) + q * kBlkW) / 4];
+    const floatx4 hi = t4[(g * (kBlkH * kBlkW) + q * kBlkW) / 4 + 1];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      a[j] = lo[j];
+      a[j + 4] = hi[j];
+    }
+    int rows[8];
+    const bool gv = g < nb - k0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) rows[j] = gv ? atox[g * kBlkW + j] : -1;
+    wave_lds_sync();
+    breg_t<DTC_PRECISION_FP32, VEC> breg[1][8];
+    gather_b<DTC_PRECISION_FP32, VEC, 1>(p, rows, colbase, breg);
+    mfma_group<DTC_PRECISION_FP32, VEC, 1>(a, breg, acc, true);
+  }
+  store_window<VEC, 1, false>(p, w, colbase, acc);
+}
+
 }  // namespace dtc
 
 using namespace dtc;
 
 extern "C" {
+
+dtc_status_t dtc_spmm_tcgnn(const int32_t *node_pointer, const int32_t *edge_list,
+                            const int32_t *block_partition, const int32_t *edge_to_column,
+                            const int32_t *edge_to_row, const float *values, int32_t num_nodes,
+                            int32_t nnz, const float *b, int32_t b_rows, int32_t n, float *c,
+                            const dtc_spmm_options_t *options, dtc_stream_t stream_) {
+  clear_error();
+  hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
+  if (num_nodes < 0 || nnz < 0 || b_rows < 0)
+    return fail(DTC_ERROR_INVALID_VALUE, "sizes must be >= 0");
+  if (n <= 0) return fail(DTC_ERROR_INVALID_VALUE, "N (feature width) must be >= 1");
+  if (options && (options->precision != DTC_PRECISION_FP32 || options->b_dtype != DTC_B_FLOAT32))
+    return fail(DTC_ERROR_UNSUPPORTED, "the TC-GNN baseline runs float32 only");
+  if (num_nodes == 0) return DTC_SUCCESS;
+  if (!node_pointer || !block_partition || !c)
+    return fail(DTC_ERROR_INVALID_VALUE, "node_pointer, block_partition and C are required");
+  if (nnz > 0 && (!edge_list || !edge_to_column || !edge_to_row))
+    return fail(DTC_ERROR_INVALID_VALUE, "edge arrays are required when nnz > 0");
+  if (b_rows > 0 && !b) return fail(DTC_ERROR_INVALID_VALUE, "B is NULL");
+  dtc_spmm_options_t o{};
+  o.precision = DTC_PRECISION_FP32;
+  o.vec = options ? options->vec : 0;
+  o.subtiles = 1;
+  Variant var;
+  dtc_status_t st = resolve_shape(&o, n, b, c, &var);
+  if (st != DTC_SUCCESS) return st;
+  const int32_t W = (int32_t)ceil_div(num_nodes, kBlkH);
+  if (b_rows == 0) return zero_output(c, num_nodes, n, stream);
+  SpmmParams p{};
+  p.B = b;
+  p.C = c;
+  p.M = num_nodes;
+  p.b_rows = b_rows;
+  p.N = n;
+  p.n_ct = (int32_t)ceil_div(n, 16 * var.vec);
+  const int64_t grid = (int64_t)p.n_ct * ceil_div(W, kWavesPerBlock);
+  if (grid > INT32_MAX) return fail(DTC_ERROR_UNSUPPORTED, "grid too large");
+  switch (var.vec) {
+    case 4:
+      spmm_tcgnn_kernel<4><<<(unsigned)grid, 256, 0, stream>>>(node_pointer, edge_list, edge_to_column,
+                                                               edge_to_row, block_partition, values, W, p);
+      break;
+    case 2:
+      spmm_tcgnn_kernel<2><<<(unsigned)grid, 256, 0, stream>>>(node_pointer, edge_list, edge_to_column,
+                                                               edge_to_row, block_partition, values, W, p);
+      break;
+    default:
+      spmm_tcgnn_kernel<1><<<(unsigned)grid, 256, 0, stream>>>(node_pointer, edge_list, edge_to_column,
+                                                               edge_to_row, block_partition, values, W, p);
+  }
+  DTC_LAUNCH_CHECK("spmm_tcgnn");
+  return DTC_SUCCESS;
+}
 
 dtc_status_t dtc_spmm_windows(const int32_t *row_window_offset, const uint8_t *tcblocktile_id,
                               const int32_t *tcblock_offset, const int32_t *sparse_atox,

@@ -168,6 +168,23 @@ dtc_status_t dtc_spmm_balanced(const int32_t *tcblock_rowid, const uint8_t *tcbl
                                float *c, const dtc_spmm_options_t *options, void *workspace,
                                size_t workspace_bytes, dtc_stream_t stream);
 
+/*
+ * TC-GNN's SpMM, the comparison baseline without TC-block compaction
+ * (replaces run_TCGNNSpMM / spmm_forward_cuda, DTCSpMM.cpp:97-115,
+ * DTCSpMM_kernel.cu:703-808).  Its format is the phase-1 output of the ME-TCF
+ * build: block_partition [ceil(num_nodes / 16)], edge_to_column and
+ * edge_to_row [nnz] over the CSR (node_pointer, edge_list), which must be a
+ * valid CSR of nnz entries.  values (CSR order) may be NULL: A = 1, as the
+ * reference's TC-GNN kernel computes.  Float32 only (options may be NULL; a
+ * precision other than FP32 is DTC_ERROR_UNSUPPORTED).  Every output element
+ * of the first num_nodes rows is written.
+ */
+dtc_status_t dtc_spmm_tcgnn(const int32_t *node_pointer, const int32_t *edge_list,
+                            const int32_t *block_partition, const int32_t *edge_to_column,
+                            const int32_t *edge_to_row, const float *values, int32_t num_nodes,
+                            int32_t nnz, const float *b, int32_t b_rows, int32_t n, float *c,
+                            const dtc_spmm_options_t *options, dtc_stream_t stream);
+
 /* ---------------------------------------------------------------- backward
  *
  * The reference binds its "backward" names to the forward functions

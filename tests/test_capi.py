@@ -172,3 +172,18 @@ def test_gather_ceiling_probe_library():
     assert probe.dtc_gather_ceiling_out_size(1000, 256, 100) == -1  # N not a multiple of 64
     assert probe.dtc_gather_ceiling_out_size(0, 256, 256) == -1
     assert probe.dtc_gather_ceiling(None, 0, 256, None, 256, None, None) == 1
+
+
+def test_tcgnn_entry_points_without_gpu(ext, lib):
+    """TC-GNN's preprocess needs the GPU build and says so; the C entry point
+    rejects a non-float32 precision before any launch."""
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    i32 = torch.int32
+    with pytest.raises(RuntimeError, match="GPU"):
+        ext.preprocess(torch.zeros(4, dtype=i32), torch.tensor([0, 4], dtype=i32), 1, 16, 8,
+                       torch.zeros(1, dtype=i32), torch.zeros(4, dtype=i32), torch.zeros(4, dtype=i32))
+    dummy = ctypes.c_void_p(16)
+    o = Opts(4, 0, 0, 0, 0, 0)  # bf16x6
+    assert lib.dtc_spmm_tcgnn(dummy, dummy, dummy, dummy, dummy, None, 16, 4, dummy, 16, 8, dummy,
+                              ctypes.byref(o), None) == 2
