@@ -60,6 +60,7 @@ struct SpmmParams {
   int32_t num_items;   // windows (unbalanced) or chunks (balanced)
   int32_t chunk;       // balanced chunk length in TC blocks
   int32_t *queue;      // persistent work counter
+  const int2 *runs;    // balanced: run [b, e) of every chunk, cut flags in the sign bits
 };
 
 // Kernels take the arrays as restrict-qualified arguments: the compiler may
@@ -74,7 +75,7 @@ struct SpmmParams {
   SpmmParams {                                                                             \
     a_rwo, a_rowid, a_tile, a_off, a_atox, a_vals, a_B, a_C, a_p.num_windows,              \
         a_p.num_blocks, a_p.M, a_p.b_rows, a_p.N, a_p.nnz, a_p.n_ct, a_p.num_items,        \
-        a_p.chunk, a_p.queue                                                               \
+        a_p.chunk, a_p.queue, a_p.runs                                                     \
   }
 
 template <int VEC>
@@ -264,39 +265,79 @@ __device__ __forceinline__ void mfma_group(const float (&a)[8], const breg_t<PRE
   }
 }
 
-// Metadata of one K=32 group (4 TC blocks), loaded one group ahead of its
-// use so that its latency hides behind the previous group's B gathers.
+// Metadata of one K=32 group (up to 4 TC blocks of one window), loaded one
+// group ahead of its use so that its latency hides behind the previous
+// group's B gathers.  Loading does not need the group's length: the offsets
+// of all 4 candidate blocks are read and `resolve_meta` trims them once the
+// length is known.
 struct GroupMeta {
-  int o0, o1, o2, o3, o4;  // element offsets of the group's blocks (wave-uniform)
+  int bb;                  // first block (wave-uniform)
+  int f0, f1, f2, f3, f4;  // TCblock_offset[bb + k], raw
+  int o0, o1, o2, o3, o4;  // element offsets of the group's blocks (resolved)
   int4 r0, r1;             // SparseAToX of block g (raw; masked where used)
-  int nblk;                // blocks in the group (wave-uniform)
-  int tile;                // tile id of element o0 + lane (raw)
+  int nblk;                // blocks in the group (resolved, wave-uniform)
+  int tile;                // tile id of element f0 + lane (raw)
   float val;               // its value (raw)
+  int rid;                 // window of block bb + lane, -1 at or past the run's end
 };
 
 // Loads are unconditional and their results are only consumed one group
-// later, so nothing in the current group waits on them.
+// later, so nothing in the current group waits on them.  With WALK the
+// lane also reads the window id of block bb + lane: the 64-block look-ahead
+// from which the group's length (it stops at the next window start) is
+// found without another round trip.
+template <bool WALK>
 __device__ __forceinline__ void load_meta(const SpmmParams &p, int bb, int b1, bool has_vals,
                                           GroupMeta &m) {
   const int lane = lane_id();
   const int g = lane >> 4;
-  const int nblk = min(kGroup, b1 - bb);
-  const int o4 = min(p.off[bb + nblk], p.nnz);
-  m.o0 = min(p.off[bb], o4);
-  m.o1 = min(p.off[bb + min(1, nblk)], o4);
-  m.o2 = min(p.off[bb + min(2, nblk)], o4);
-  m.o3 = min(p.off[bb + min(3, nblk)], o4);
-  m.o4 = o4;
-  m.nblk = nblk;
+  const int T = p.num_blocks;
+  m.bb = bb;
+  m.f0 = p.off[bb];
+  m.f1 = p.off[min(bb + 1, T)];
+  m.f2 = p.off[min(bb + 2, T)];
+  m.f3 = p.off[min(bb + 3, T)];
+  m.f4 = p.off[min(bb + 4, T)];
   const int blk = min(bb + g, b1 - 1);
   const int4 *ap = reinterpret_cast<const int4 *>(p.atox + (int64_t)blk * kBlkW);
   m.r0 = ap[0];
   m.r1 = ap[1];
   // The first 64 stored elements of the group, one per lane (a group holds
   // 36 on average for R-MAT; more than 64 take the slow path in build_a).
-  const int ec = min(m.o0 + lane, p.nnz - 1);
+  const int e0 = min(max(m.f0, 0), p.nnz);
+  const int ec = e0 < p.nnz - lane ? e0 + lane : p.nnz - 1;
   m.tile = p.tile[ec];
   m.val = has_vals ? p.vals[ec] : 1.0f;
+  if constexpr (WALK) {
+    const int rb = bb + lane;
+    m.rid = rb < b1 ? p.rowid[rb] : -1;
+  }
+}
+
+// Group length and element offsets.  WALK: the group ends before the first
+// block of the look-ahead whose window differs from its predecessor's (or
+// at the run's end, where the look-ahead holds -1); else at b1.
+template <bool WALK>
+__device__ __forceinline__ void resolve_meta(const SpmmParams &p, int b1, GroupMeta &m) {
+  int nblk;
+  if constexpr (WALK) {
+    const int prev = __shfl_up(m.rid, 1, 64);
+    const uint64_t starts = __ballot(lane_id() > 0 && m.rid != prev);
+    nblk = starts ? min(kGroup, (int)__builtin_ctzll(starts)) : kGroup;
+  } else {
+    nblk = min(kGroup, b1 - m.bb);
+  }
+  // selects, not an indexed array (which would live in scratch memory)
+  auto f = [&m](int k) {
+    return k == 0 ? m.f0 : k == 1 ? m.f1 : k == 2 ? m.f2 : k == 3 ? m.f3 : m.f4;
+  };
+  const int o4 = min(f(nblk), p.nnz);
+  m.o0 = min(m.f0, o4);
+  m.o1 = min(f(min(1, nblk)), o4);
+  m.o2 = min(f(min(2, nblk)), o4);
+  m.o3 = min(f(min(3, nblk)), o4);
+  m.o4 = o4;
+  m.nblk = nblk;
 }
 
 // This lane's 8 B rows (-1 for lanes whose block lies past the run).
@@ -384,7 +425,8 @@ __device__ __forceinline__ void run_blocks(const SpmmParams &p, int b0, int b1, 
                                            bool has_vals) {
   if (b0 >= b1) return;
   GroupMeta cur;
-  load_meta(p, b0, b1, has_vals, cur);
+  load_meta<false>(p, b0, b1, has_vals, cur);
+  resolve_meta<false>(p, b1, cur);
   for (int bb = b0; bb < b1; bb += kGroup) {
     breg_t<PREC, VEC> breg[NSUB][8];
     int rows[8];
@@ -393,11 +435,12 @@ __device__ __forceinline__ void run_blocks(const SpmmParams &p, int b0, int b1, 
     // The last group re-reads its own metadata instead of branching.
     const int bn = (bb + kGroup < b1) ? bb + kGroup : bb;
     GroupMeta nxt;
-    load_meta(p, bn, b1, has_vals, nxt);
+    load_meta<false>(p, bn, b1, has_vals, nxt);
     float a[8];
     build_a(p, cur, has_vals, lds, a);
     mfma_group<PREC, VEC, NSUB>(a, breg, acc, has_vals);
     cur = nxt;
+    resolve_meta<false>(p, b1, cur);
   }
 }
 
@@ -441,6 +484,52 @@ __device__ __forceinline__ void store_window(const SpmmParams &p, int w, int col
   }
 }
 
+// Accumulates the TC blocks [b, e) of the balanced schedule, which may span
+// many windows, as ONE software pipeline: groups never cross a window start
+// (they are the groups a per-window loop would form), and when a group opens
+// a new window the accumulators of the previous one are stored and cleared.
+// Window `atomic_w0` (if >= 0) and `atomic_w1` (if >= 0) are merged with
+// atomic adds (windows cut at chunk boundaries), every other window is
+// stored whole.  Restarting the pipeline per window instead costs one
+// metadata round trip per window, and most windows of skewed graphs hold
+// only a few blocks.
+template <int PREC, int VEC, int NSUB>
+__device__ __forceinline__ void walk_blocks(const SpmmParams &p, int b, int e, int colbase,
+                                            float *lds, floatx4 (&acc)[NSUB][VEC],
+                                            bool has_vals, int atomic_w0, int atomic_w1) {
+  GroupMeta cur;
+  load_meta<true>(p, b, e, has_vals, cur);
+  resolve_meta<true>(p, e, cur);
+  int cw = wave_uniform(cur.rid);
+  zero_acc<VEC, NSUB>(acc);
+  while (true) {
+    const int w = wave_uniform(cur.rid);  // window of block cur.bb (lane 0)
+    if (w != cw) {
+      if (cw == atomic_w0 || cw == atomic_w1) store_window<VEC, NSUB, true>(p, cw, colbase, acc);
+      else store_window<VEC, NSUB, false>(p, cw, colbase, acc);
+      zero_acc<VEC, NSUB>(acc);
+      cw = w;
+    }
+    breg_t<PREC, VEC> breg[NSUB][8];
+    int rows[8];
+    meta_rows(cur, rows);
+    gather_b<PREC, VEC, NSUB>(p, rows, colbase, breg);
+    const int bn = cur.bb + cur.nblk;
+    const bool more = bn < e;
+    // The last group re-reads its own metadata instead of branching.
+    GroupMeta nxt;
+    load_meta<true>(p, more ? bn : cur.bb, e, has_vals, nxt);
+    float a[8];
+    build_a(p, cur, has_vals, lds, a);
+    mfma_group<PREC, VEC, NSUB>(a, breg, acc, has_vals);
+    if (!more) break;
+    cur = nxt;
+    resolve_meta<true>(p, e, cur);
+  }
+  if (cw == atomic_w0 || cw == atomic_w1) store_window<VEC, NSUB, true>(p, cw, colbase, acc);
+  else store_window<VEC, NSUB, false>(p, cw, colbase, acc);
+}
+
 // End of window w whose block b is known: one ballot over the next 64 block
 // ids covers most windows; longer ones finish with a wave-parallel search.
 __device__ __forceinline__ int window_end_from(const int32_t *rowid, int b, int T, int w) {
@@ -474,35 +563,17 @@ __device__ __forceinline__ void run_unit(const SpmmParams &p, int item, int ct, 
     run_blocks<PREC, VEC, NSUB>(p, b0, b1, colbase, lds, acc, has_vals);
     store_window<VEC, NSUB, false>(p, w, colbase, acc);
   } else {
-    // A window of at most `chunk` blocks belongs whole to the chunk holding
-    // its first block (and may run past that chunk's end); only longer
-    // windows are cut at chunk boundaries and merged with atomic adds into
-    // rows the preparation kernel zeroed.
-    const int T = p.num_blocks;
-    const int c0 = item * p.chunk;
-    const int c1 = min(c0 + p.chunk, T);
-    int b = c0;
-    while (b < c1) {
-      const int w = wave_uniform(p.rowid[b]);
-      const int ws = (b == c0) ? wave_uniform(window_start_from(p.rowid, b, w)) : b;
-      const int we = wave_uniform(max(b + 1, window_end_from(p.rowid, b, T, w)));
-      if (we - ws <= p.chunk) {
-        if (ws < c0) {
-          b = we;
-          continue;
-        }
-        zero_acc<VEC, NSUB>(acc);
-        run_blocks<PREC, VEC, NSUB>(p, b, we, colbase, lds, acc, has_vals);
-        store_window<VEC, NSUB, false>(p, w, colbase, acc);
-        b = we;
-      } else {
-        const int se = min(we, c1);
-        zero_acc<VEC, NSUB>(acc);
-        run_blocks<PREC, VEC, NSUB>(p, b, se, colbase, lds, acc, has_vals);
-        store_window<VEC, NSUB, true>(p, w, colbase, acc);
-        b = se;
-      }
-    }
+    // The chunk's run and the windows it merges atomically were worked out
+    // by the preparation kernel (k_balance_prep).
+    const int2 r = p.runs[item];
+    const bool cut_first = r.x < 0;
+    const bool cut_last = r.y < 0;
+    const int b = cut_first ? ~r.x : r.x;
+    const int e = cut_last ? ~r.y : r.y;
+    if (b >= e) return;
+    const int w_first = cut_first ? wave_uniform(p.rowid[b]) : -1;
+    const int w_last = cut_last ? wave_uniform(p.rowid[e - 1]) : -1;
+    walk_blocks<PREC, VEC, NSUB>(p, b, e, colbase, lds, acc, has_vals, w_first, w_last);
   }
 }
 
@@ -539,21 +610,12 @@ __global__ __launch_bounds__(256) void spmm_kernel_persistent(SPMM_KERNEL_ARGS) 
   }
 }
 
-// Balanced-schedule preparation, one wave per window: zero the rows of
-// windows no chunk stores whole (empty windows and windows longer than a
-// chunk) and reset the work counter.
-__global__ __launch_bounds__(256) void k_balance_prep(const int32_t *rowid, int32_t T,
-                                                      int32_t W, int32_t M, int32_t N,
-                                                      int32_t chunk, float *C, int32_t *queue) {
-  if (queue && blockIdx.x == 0 && threadIdx.x == 0) *queue = 0;
-  const int lane = threadIdx.x & 63;
-  const int64_t w = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-  if (w >= W) return;
-  const int fb = wave_lower_bound<int32_t>(rowid, 0, T, (int32_t)w);
-  const int lb = wave_lower_bound<int32_t>(rowid, fb, T, (int32_t)(w + 1));
-  if (fb < lb && lb - fb <= chunk) return;
+// Zeroes the rows of window w (one wave).
+__device__ __forceinline__ void zero_window_rows(float *C, int64_t w, int32_t M, int32_t N) {
+  const int lane = lane_id();
   const int64_t r0 = w * kBlkH;
   const int64_t r1 = (r0 + kBlkH < (int64_t)M) ? r0 + kBlkH : (int64_t)M;
+  if (r0 >= r1) return;
   float *base = C + r0 * (int64_t)N;
   const int64_t count = (r1 - r0) * (int64_t)N;
   if ((N & 3) == 0 && ((uintptr_t)base & 15) == 0) {
@@ -563,6 +625,77 @@ __global__ __launch_bounds__(256) void k_balance_prep(const int32_t *rowid, int3
   } else {
     for (int64_t idx = lane; idx < count; idx += 64) base[idx] = 0.f;
   }
+}
+
+constexpr int kPrepWindowsPerWave = 8;
+
+// Balanced-schedule preparation.  Ownership rule: a window of at most `chunk`
+// blocks belongs whole to the chunk holding its first block (and may run
+// past that chunk's end); longer windows are cut at chunk boundaries and
+// their pieces merged with atomic adds.  Waves v < num_items take chunk v,
+// the following waves 8 windows each, so that neither kind waits for the
+// other's searches.  Wave v = num_items + u
+//   * zeroes the empty windows among [8u, 8u + 8) (a window id absent from
+//     rowid, found by one search and then one look-ahead per window).
+// Wave v < num_items
+//   * for chunk v ([c0, c1) = blocks [v*chunk, (v+1)*chunk)), writes the
+//     chunk's run [b, e) to runs[v]: b = c0 unless c0 lies inside a short
+//     window that began earlier (then b = that window's end), e = c1 unless
+//     a short window that began inside the chunk runs past c1 (then e = that
+//     window's end); ~b marks a cut first window, ~e a cut last window (b >= e:
+//     nothing to do); and zeroes the rows of the cut windows that start in the
+//     chunk, since no chunk stores them whole;
+//   * (wave 0) resets the persistent work counter.
+// The window searches run here, once per chunk, on an idle memory system
+// instead of at the start of every SpMM wave.
+__global__ __launch_bounds__(256) void k_balance_prep(const int32_t *rowid, int32_t T,
+                                                      int32_t W, int32_t M, int32_t N,
+                                                      int32_t chunk, int32_t num_items,
+                                                      float *C, int32_t *queue, int2 *runs) {
+  if (queue && blockIdx.x == 0 && threadIdx.x == 0) *queue = 0;
+  const int lane = lane_id();
+  const int64_t v = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (v >= num_items) {
+    const int64_t wa = (v - num_items) * kPrepWindowsPerWave;
+    if (wa >= W) return;
+    const int wb = (int)(wa + kPrepWindowsPerWave < W ? wa + kPrepWindowsPerWave : W);
+    int lb = wave_uniform(wave_lower_bound<int32_t>(rowid, 0, T, (int32_t)wa));
+    for (int w = (int)wa; w < wb; ++w) {
+      // nx = first block with id > w; blocks of window w are [lb, nx)
+      const int idx = lb + lane;
+      const uint64_t m = __ballot(idx >= T || rowid[idx] > w);
+      const int nx = wave_uniform(m ? lb + (int)__builtin_ctzll(m)
+                                    : wave_lower_bound<int32_t>(rowid, lb + 64, T, w + 1));
+      if (nx == lb) zero_window_rows(C, w, M, N);
+      lb = nx;
+    }
+    return;
+  }
+  const int c0 = (int)v * chunk;
+  const int c1 = min(c0 + chunk, T);
+  const int w0 = wave_uniform(rowid[c0]);
+  const int ws0 = wave_uniform(window_start_from(rowid, c0, w0));
+  const int we0 = wave_uniform(max(c0 + 1, window_end_from(rowid, c0, T, w0)));
+  const bool first_long = we0 - ws0 > chunk;
+  int b = (!first_long && ws0 < c0) ? we0 : c0;
+  int e = c1;
+  bool last_long = false;
+  int wl = -1, wsl = 0;
+  if (c1 < T) {
+    const int r_in = wave_uniform(rowid[c1 - 1]);
+    const int r_out = wave_uniform(rowid[c1]);
+    if (r_in == r_out) {  // a window crosses c1
+      wl = r_in;
+      wsl = r_in == w0 ? ws0 : wave_uniform(window_start_from(rowid, c1, r_in));
+      const int wel = r_in == w0 ? we0 : wave_uniform(max(c1 + 1, window_end_from(rowid, c1, T, r_in)));
+      if (wel - wsl > chunk) last_long = true;
+      else e = wel;
+    }
+  }
+  if (b >= c1) b = e = c1;
+  if (lane == 0) runs[v] = make_int2(first_long ? ~b : b, last_long ? ~e : e);
+  if (first_long && ws0 == c0) zero_window_rows(C, w0, M, N);
+  if (last_long && wsl >= c0 && !(wl == w0 && ws0 == c0)) zero_window_rows(C, wl, M, N);
 }
 
 // ------------------------------------------------------------ host side
@@ -675,6 +808,13 @@ dtc_status_t check_common(const uint8_t *tile, const int32_t *off, const int32_t
   if ((int64_t)num_nodes * n > ((int64_t)1 << 40))
     return fail(DTC_ERROR_UNSUPPORTED, "output too large");
   return DTC_SUCCESS;
+}
+
+// Balanced workspace: the persistent work counter, then one int2 run per
+// chunk (at most one chunk per block).
+constexpr size_t kQueueBytes = 256;
+size_t balanced_workspace_bytes(int32_t num_blocks) {
+  return kQueueBytes + sizeof(int2) * (size_t)std::max<int32_t>(num_blocks, 0);
 }
 
 __global__ __launch_bounds__(256) void k_zero(float *c, int64_t count) {
@@ -878,8 +1018,8 @@ dtc_status_t dtc_spmm_windows(const int32_t *row_window_offset, const uint8_t *t
 dtc_status_t dtc_spmm_balanced_workspace_size(int32_t num_blocks, size_t *bytes) {
   clear_error();
   if (!bytes) return fail(DTC_ERROR_INVALID_VALUE, "bytes is NULL");
-  (void)num_blocks;
-  *bytes = 256;
+  if (num_blocks < 0) return fail(DTC_ERROR_INVALID_VALUE, "num_blocks must be >= 0");
+  *bytes = balanced_workspace_bytes(num_blocks);
   return DTC_SUCCESS;
 }
 
@@ -901,11 +1041,12 @@ dtc_status_t dtc_spmm_balanced(const int32_t *tcblock_rowid, const uint8_t *tcbl
   if (st != DTC_SUCCESS) return st;
   const int chunk = (options && options->chunk_blocks > 0) ? options->chunk_blocks : 256;
   const bool persistent = options && options->persistent;
-  if (persistent && (!workspace || workspace_bytes < 256))
-    return fail(DTC_ERROR_WORKSPACE, "persistent balanced SpMM needs a 256-byte workspace");
   const int32_t W = (int32_t)ceil_div(num_nodes, kBlkH);
   if (W == 0) return DTC_SUCCESS;
-  if (b_rows == 0) return zero_output(c, num_nodes, n, stream);
+  if (num_blocks > 0 && (!workspace || workspace_bytes < balanced_workspace_bytes(num_blocks)))
+    return fail(DTC_ERROR_WORKSPACE,
+                "balanced SpMM needs the workspace that dtc_spmm_balanced_workspace_size reports");
+  if (b_rows == 0 || num_blocks == 0) return zero_output(c, num_nodes, n, stream);
   SpmmParams p{};
   p.rowid = tcblock_rowid;
   p.tile = tcblocktile_id;
@@ -924,10 +1065,12 @@ dtc_status_t dtc_spmm_balanced(const int32_t *tcblock_rowid, const uint8_t *tcbl
   p.num_items = (int32_t)ceil_div(num_blocks, chunk);
   p.chunk = chunk;
   p.queue = persistent ? static_cast<int32_t *>(workspace) : nullptr;
-  k_balance_prep<<<(unsigned)ceil_div(W, kWavesPerBlock), 256, 0, stream>>>(
-      tcblock_rowid, num_blocks, W, num_nodes, n, chunk, c, p.queue);
+  int2 *runs = reinterpret_cast<int2 *>(static_cast<char *>(workspace) + kQueueBytes);
+  p.runs = runs;
+  const int64_t prep_waves = (int64_t)p.num_items + ceil_div(W, kPrepWindowsPerWave);
+  k_balance_prep<<<(unsigned)ceil_div(prep_waves, kWavesPerBlock), 256, 0, stream>>>(
+      tcblock_rowid, num_blocks, W, num_nodes, n, chunk, p.num_items, c, p.queue, runs);
   DTC_LAUNCH_CHECK("balance_prep");
-  if (num_blocks == 0) return DTC_SUCCESS;
   if (!persistent) {
     KernelFn fn = pick_kernel(var, 1);
     const int64_t grid = (int64_t)p.n_ct * ceil_div(p.num_items, kWavesPerBlock);

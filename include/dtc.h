@@ -158,7 +158,10 @@ dtc_status_t dtc_spmm_windows(const int32_t *row_window_offset, const uint8_t *t
  * chunks are merged with float32 atomic adds into rows this call zeroes
  * first.  Replaces run_DTCSpMM_balance /
  * spmm_balance_forward_cuda_ptx_unit8_prefetch (DTCSpMM.cpp:163-178,
- * DTCSpMM_kernel.cu:1118-1199).  Every output element is written.
+ * DTCSpMM_kernel.cu:1118-1199).  Every output element is written.  The
+ * workspace (required when num_blocks > 0; 256 + 8 * num_blocks bytes) holds
+ * each chunk's block run, worked out by a preparation kernel, and the work
+ * counter of the persistent variant.
  */
 dtc_status_t dtc_spmm_balanced_workspace_size(int32_t num_blocks, size_t *bytes);
 dtc_status_t dtc_spmm_balanced(const int32_t *tcblock_rowid, const uint8_t *tcblocktile_id,

@@ -111,11 +111,20 @@ def test_host_side_argument_errors(lib):
     st = lib.dtc_metcf_build_windows(dummy, dummy, 16, 4, 16, 8, dummy, dummy, dummy, dummy,
                                      ctypes.byref(ctypes.c_int32()), None, 0, None)
     assert st == 3
-    # persistent balanced launch without its workspace
-    o = Opts(0, 0, 0, 64, 1)
-    st = lib.dtc_spmm_balanced(dummy, dummy, dummy, dummy, None, 4, 8, 16, dummy, 16, 8, dummy, ctypes.byref(o),
-                               None, 0, None)
-    assert st == 3
+    # balanced launches without their workspace (persistent or not), or with
+    # one smaller than the size query reports
+    size = ctypes.c_size_t()
+    assert lib.dtc_spmm_balanced_workspace_size(4, ctypes.byref(size)) == 0
+    assert size.value == 256 + 8 * 4
+    assert lib.dtc_spmm_balanced_workspace_size(-1, ctypes.byref(size)) == 1
+    for pers in (0, 1):
+        o = Opts(0, 0, 0, 64, pers)
+        st = lib.dtc_spmm_balanced(dummy, dummy, dummy, dummy, None, 4, 8, 16, dummy, 16, 8, dummy,
+                                   ctypes.byref(o), None, 0, None)
+        assert st == 3
+        st = lib.dtc_spmm_balanced(dummy, dummy, dummy, dummy, None, 4, 8, 16, dummy, 16, 8, dummy,
+                                   ctypes.byref(o), dummy, 256 + 8 * 3, None)
+        assert st == 3 and b"workspace" in lib.dtc_last_error()
 
 
 @pytest.fixture(scope="module")

@@ -128,6 +128,43 @@ def test_balanced_chunks(ext, ragged, rmat12, chunk, persistent):
         assert componentwise(C, C64, Cabs) <= TOL["fp32"], (case.name, chunk, persistent)
 
 
+@pytest.fixture(scope="module")
+def gappy(ext):
+    """Long runs of empty row windows (at the start, in the middle, at the
+    end) around windows short and long, so that chunks begin and end inside
+    gaps, inside windows cut at chunk boundaries and on window starts."""
+    rng = np.random.default_rng(33)
+    M = 3000 + 5
+    rows = [[] for _ in range(M)]
+    for r in list(range(400, 1000)) + list(range(2200, 2900)):
+        if rng.random() < 0.7:
+            rows[r] = sorted(rng.choice(M, int(rng.integers(1, 12)), replace=False).tolist())
+    for r in (401, 950, 2201, 2899):  # long rows: windows of many blocks
+        rows[r] = sorted(rng.choice(M, 900, replace=False).tolist())
+    rowptr = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int32)
+    col = np.concatenate([np.array(r, np.int32) for r in rows]).astype(np.int32)
+    vals = rng.uniform(-1, 1, col.shape[0]).astype(np.float32)
+    return Case(ext, rowptr, col, M, vals, "gappy")
+
+
+@pytest.mark.parametrize("chunk", [1, 3, 8, 64, 113, 1000, 100000])
+@pytest.mark.parametrize("persistent", [False, True])
+def test_balanced_empty_windows(ext, gappy, chunk, persistent):
+    # The output starts as NaN: every element must be written (zeros for the
+    # empty windows, sums elsewhere), whatever the chunking.
+    rng = np.random.default_rng(chunk)
+    Bn = rng.standard_normal((gappy.M, 64)).astype(np.float32)
+    B = torch.from_numpy(Bn).cuda()
+    C64, Cabs = gappy.ref(Bn)
+    out = torch.full((gappy.M, 64), float("nan"), device="cuda")
+    say("spmm gappy", chunk, persistent)
+    gappy.run(ext, B, "balanced", "fp32", chunk_blocks=chunk, persistent=persistent, out=out)
+    C = out.cpu().numpy()
+    assert np.isfinite(C).all()
+    assert (C[:400] == 0).all() and (C[1000:2200] == 0).all() and (C[2900:] == 0).all()
+    assert componentwise(C, C64, Cabs) <= TOL["fp32"], (chunk, persistent)
+
+
 def test_schedules_agree_rmat(ext, rmat12):
     rng = np.random.default_rng(5)
     Bn = rng.standard_normal((rmat12.M, 128)).astype(np.float32)
