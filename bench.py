@@ -145,17 +145,17 @@ def bench_gcn(args):
     }), flush=True)
 
 
-def gather_ceiling(A, B, chunk, stream, steps):
+def gather_ceiling(A, B, chunk, vec, stream, steps):
     """Time of the gather-only probe (libdtc_probe.so): the SpMM's B-row loads
-    for this ME-TCF and chunk size, nothing else -- the practical ceiling of
-    the access pattern.  None when N is not a multiple of 64 or the probe
-    library is not built."""
+    for this ME-TCF, chunk size and column-tile width (vec 2 or 4), nothing
+    else -- the practical ceiling of the access pattern.  None when N is not
+    a multiple of the tile width or the probe library is not built."""
     import ctypes
 
     import torch
     path = os.path.join(PKG, "libdtc_probe.so")
     N = int(B.shape[1])
-    if N % 64 or not os.path.exists(path):
+    if vec not in (2, 4) or N % (16 * vec) or not os.path.exists(path):
         return None
     lib = ctypes.CDLL(path)
     lib.dtc_gather_ceiling_out_size.restype = ctypes.c_int64
@@ -164,11 +164,11 @@ def gather_ceiling(A, B, chunk, stream, steps):
     # every slot must index a row of B before the probe reads it
     if T == 0 or int(atox.min()) < 0 or int(atox.max()) >= B.shape[0] or atox.numel() != 8 * T:
         return None
-    out = torch.empty(lib.dtc_gather_ceiling_out_size(T, chunk, N), device=B.device)
+    out = torch.empty(lib.dtc_gather_ceiling_out_size(T, chunk, N, vec), device=B.device)
 
     def run():
         rc = lib.dtc_gather_ceiling(ctypes.c_void_p(atox.data_ptr()), T, chunk, ctypes.c_void_p(B.data_ptr()), N,
-                                    ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(stream.cuda_stream))
+                                    vec, ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(stream.cuda_stream))
         if rc != 0:
             raise RuntimeError(f"gather probe failed ({rc})")
     run()
@@ -480,8 +480,8 @@ def main():
 
     # ---- practical ceiling: the same B-row loads with nothing else
     ceiling = None
-    if chosen_v[0] == "balanced" and chosen_v[3] == 4:
-        g_ms = gather_ceiling(A, B, chosen_v[2], stream, args.steps)
+    if chosen_v[0] == "balanced" and chosen_v[3] in (2, 4) and chosen_v[4] == 1:
+        g_ms = gather_ceiling(A, B, chosen_v[2], chosen_v[3], stream, args.steps)
         if g_ms:
             ceiling = {"gather_only_ms": round(g_ms, 4), "spmm_frac_of_gather_ceiling": round(g_ms / ms, 4)}
 

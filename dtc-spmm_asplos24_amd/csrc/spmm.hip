@@ -503,6 +503,13 @@ __device__ __forceinline__ void walk_blocks(const SpmmParams &p, int b, int e, i
   int cw = wave_uniform(cur.rid);
   zero_acc<VEC, NSUB>(acc);
   while (true) {
+    breg_t<PREC, VEC> breg[NSUB][8];
+    int rows[8];
+    meta_rows(cur, rows);
+    gather_b<PREC, VEC, NSUB>(p, rows, colbase, breg);
+    // The previous window's stores go out after this group's gathers: loads
+    // and stores retire through one in-order counter, so a store issued first
+    // would hold up the wait for the gathers.
     const int w = wave_uniform(cur.rid);  // window of block cur.bb (lane 0)
     if (w != cw) {
       if (cw == atomic_w0 || cw == atomic_w1) store_window<VEC, NSUB, true>(p, cw, colbase, acc);
@@ -510,10 +517,6 @@ __device__ __forceinline__ void walk_blocks(const SpmmParams &p, int b, int e, i
       zero_acc<VEC, NSUB>(acc);
       cw = w;
     }
-    breg_t<PREC, VEC> breg[NSUB][8];
-    int rows[8];
-    meta_rows(cur, rows);
-    gather_b<PREC, VEC, NSUB>(p, rows, colbase, breg);
     const int bn = cur.bb + cur.nblk;
     const bool more = bn < e;
     // The last group re-reads its own metadata instead of branching.

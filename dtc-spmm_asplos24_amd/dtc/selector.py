@@ -36,24 +36,28 @@ VARIANTS = [
     ("windows", "fp32", 0, 4, 1, False),
     ("windows", "fp32", 0, 4, 2, False),
     ("windows", "fp32", 0, 2, 1, False),
-    ("balanced", "fp32", 32, 4, 1, False),
+    ("windows", "bf16x6", 0, 4, 1, False),
+    ("balanced", "fp32", 32, 2, 1, False),
+    ("balanced", "fp32", 64, 1, 1, False),
+    ("balanced", "fp32", 64, 2, 1, False),
     ("balanced", "fp32", 64, 4, 1, False),
+    ("balanced", "fp32", 128, 2, 1, False),
     ("balanced", "fp32", 128, 4, 1, False),
+    ("balanced", "fp32", 256, 2, 1, False),
     ("balanced", "fp32", 256, 4, 1, False),
+    ("balanced", "fp32", 512, 2, 1, False),
     ("balanced", "fp32", 512, 4, 1, False),
-    ("balanced", "fp32", 1024, 4, 1, False),
     ("balanced", "fp32", 256, 4, 2, False),
     ("balanced", "fp32", 1024, 4, 2, False),
-    ("balanced", "fp32", 256, 2, 1, False),
     ("balanced", "fp32", 256, 4, 1, True),
+    ("balanced", "bf16x6", 64, 2, 1, False),
     ("balanced", "bf16x6", 64, 4, 1, False),
+    ("balanced", "bf16x6", 256, 2, 1, False),
     ("balanced", "bf16x6", 256, 4, 1, False),
-    ("balanced", "bf16x6", 1024, 4, 1, False),
-    ("windows", "bf16x6", 0, 4, 1, False),
 ]
-# The method without the selector: the best fixed variant on the power-law
-# workload (balanced, FP32, 256-block chunks, 64-column tiles).
-DEFAULT_VARIANT = 6
+# The method without the selector: the library's default options (balanced,
+# FP32, 256-block chunks, 64-column tiles).
+DEFAULT_VARIANT = 11
 
 FEATURE_NAMES = [
     "log2_rows", "log2_nnz", "log2_N", "log2_avg_deg", "deg_cv", "log2_max_deg",

@@ -177,10 +177,14 @@ def test_gather_ceiling_probe_library():
         pytest.skip("libdtc_probe.so not built")
     probe = ctypes.CDLL(path)
     probe.dtc_gather_ceiling_out_size.restype = ctypes.c_int64
-    assert probe.dtc_gather_ceiling_out_size(1000, 256, 256) == 4 * 1 * 256
-    assert probe.dtc_gather_ceiling_out_size(1000, 256, 100) == -1  # N not a multiple of 64
-    assert probe.dtc_gather_ceiling_out_size(0, 256, 256) == -1
-    assert probe.dtc_gather_ceiling(None, 0, 256, None, 256, None, None) == 1
+    # 1000 blocks in 4 chunks of 256 -> one workgroup per column tile
+    assert probe.dtc_gather_ceiling_out_size(1000, 256, 256, 4) == 4 * 1 * 256
+    assert probe.dtc_gather_ceiling_out_size(1000, 256, 256, 2) == 8 * 1 * 256
+    assert probe.dtc_gather_ceiling_out_size(1000, 256, 96, 2) == 3 * 1 * 256
+    assert probe.dtc_gather_ceiling_out_size(1000, 256, 96, 4) == -1  # N not a multiple of 64
+    assert probe.dtc_gather_ceiling_out_size(1000, 256, 256, 1) == -1  # vec 1 not modelled
+    assert probe.dtc_gather_ceiling_out_size(0, 256, 256, 4) == -1
+    assert probe.dtc_gather_ceiling(None, 0, 256, None, 256, 4, None, None) == 1
 
 
 def test_tcgnn_entry_points_without_gpu(ext, lib):
