@@ -75,10 +75,12 @@ void parse_exeplan(const std::string &plan, dtc_spmm_options_t &o) {
                     "' (float_nonsplit, float2_nonsplit, float2_split, float4_nonsplit, float4_split, auto)");
 }
 
-// The widest vector width N and the base addresses allow, capped at `want`.
-int32_t fit_vec(int32_t want, int64_t n, const Tensor &b, const Tensor &c) {
+// The widest vector width N and the base addresses allow, capped at `want`
+// (0: at the library's default width for the schedule, `dflt`).
+int32_t fit_vec(int32_t want, int64_t n, const Tensor &b, const Tensor &c, int32_t dflt) {
+  const int32_t cap = want != 0 ? want : dflt;
   for (int v : {4, 2, 1}) {
-    if (want != 0 && v > want) continue;
+    if (v > cap) continue;
     const bool ok = n % v == 0 && reinterpret_cast<uintptr_t>(b.data_ptr()) % (4 * v) == 0 &&
                     reinterpret_cast<uintptr_t>(c.data_ptr()) % (4 * v) == 0;
     if (ok) return v;
@@ -333,7 +335,7 @@ std::vector<Tensor> run_windows(const Operands &o, const Tensor &rwo, int64_t nu
   Tensor c = output_for(o.input, num_nodes, ref_shape, out, &m_eff);
   const int64_t n = o.input.size(1);
   if (o.input.scalar_type() == torch::kBFloat16) opt.b_dtype = DTC_B_BFLOAT16;
-  else opt.vec = fit_vec(opt.vec, n, o.input, c);
+  else opt.vec = fit_vec(opt.vec, n, o.input, c, 4);
   // The reference bounds B rows by num_nodes (dense_bound,
   // DTCSpMM_kernel.cu:2687); the general entry takes B's own row count.
   const int64_t b_rows = ref_shape ? std::min<int64_t>(num_nodes, o.input.size(0)) : o.input.size(0);
@@ -364,7 +366,7 @@ std::vector<Tensor> run_balanced(const Operands &o, const Tensor &rowid, int64_t
   Tensor c = output_for(o.input, num_nodes, ref_shape, out, &m_eff);
   const int64_t n = o.input.size(1);
   if (o.input.scalar_type() == torch::kBFloat16) opt.b_dtype = DTC_B_BFLOAT16;
-  else opt.vec = fit_vec(opt.vec, n, o.input, c);
+  else opt.vec = fit_vec(opt.vec, n, o.input, c, 2);
   // The reference bounds B rows by num_nodes (dense_bound,
   // DTCSpMM_kernel.cu:2687); the general entry takes B's own row count.
   const int64_t b_rows = ref_shape ? std::min<int64_t>(num_nodes, o.input.size(0)) : o.input.size(0);

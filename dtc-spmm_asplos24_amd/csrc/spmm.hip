@@ -39,6 +39,13 @@ constexpr int kBlkH = 16;
 constexpr int kBlkW = 8;
 constexpr int kGroup = 4;                        // TC blocks per K=32 group
 constexpr int kTileFloats = kGroup * kBlkH * kBlkW;  // 512
+
+// Per-wave LDS floats of a kernel instance: the A tiles of one group, or the
+// 16-row output tile of one window while it is merged with atomics.
+template <int VEC, int NSUB>
+constexpr int wave_lds_floats() {
+  return kBlkH * 16 * VEC * NSUB > kTileFloats ? kBlkH * 16 * VEC * NSUB : kTileFloats;
+}
 constexpr int kWavesPerBlock = 4;
 
 struct SpmmParams {
@@ -452,8 +459,8 @@ __device__ __forceinline__ void zero_acc(floatx4 (&acc)[NSUB][VEC]) {
     for (int v = 0; v < VEC; ++v) acc[s][v] = floatx4{0.f, 0.f, 0.f, 0.f};
 }
 
-// Writes (or atomically adds) the 16 rows of window w held in acc.
-template <int VEC, int NSUB, bool ATOMIC>
+// Writes the 16 rows of window w held in acc.
+template <int VEC, int NSUB>
 __device__ __forceinline__ void store_window(const SpmmParams &p, int w, int colbase,
                                              const floatx4 (&acc)[NSUB][VEC]) {
   typedef typename VecType<VEC>::type vfloat;
@@ -468,20 +475,48 @@ __device__ __forceinline__ void store_window(const SpmmParams &p, int w, int col
     for (int s = 0; s < NSUB; ++s) {
       const int col = colbase + s * 16 * VEC;
       if (col >= p.N) continue;
-      float *dst = p.C + (int64_t)row * N + col;
-      if constexpr (ATOMIC) {
+      vfloat out;
+      if constexpr (VEC == 1) out = acc[s][0][i];
+      else
 #pragma unroll
-        for (int v = 0; v < VEC; ++v) unsafeAtomicAdd(dst + v, acc[s][v][i]);
-      } else {
-        vfloat out;
-        if constexpr (VEC == 1) out = acc[s][0][i];
-        else
-#pragma unroll
-          for (int v = 0; v < VEC; ++v) out[v] = acc[s][v][i];
-        __builtin_nontemporal_store(out, reinterpret_cast<vfloat *>(dst));
-      }
+        for (int v = 0; v < VEC; ++v) out[v] = acc[s][v][i];
+      __builtin_nontemporal_store(out, reinterpret_cast<vfloat *>(p.C + row * N + col));
     }
   }
+}
+
+// Adds the 16 rows of window w held in acc to C with atomics, through this
+// wave's LDS: the tile is written in the accumulator layout and read back
+// row-major, so that each atomic instruction covers 64 consecutive floats of
+// a row (or two rows of a 32-column tile) instead of lanes 16 bytes apart
+// spread over 4 rows -- a quarter of the cache lines per instruction.
+template <int VEC, int NSUB>
+__device__ __forceinline__ void add_window_atomic(const SpmmParams &p, int w, int colbase,
+                                                  float *lds, const floatx4 (&acc)[NSUB][VEC]) {
+  constexpr int TW = 16 * VEC * NSUB;  // tile width in floats
+  const int lane = lane_id();
+  const int g = lane >> 4;
+  const int q = lane & 15;
+#pragma unroll
+  for (int s = 0; s < NSUB; ++s)
+#pragma unroll
+    for (int v = 0; v < VEC; ++v)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) lds[(4 * g + i) * TW + s * 16 * VEC + q * VEC + v] = acc[s][v][i];
+  wave_lds_sync();
+  const int col0 = colbase - q * VEC;  // first column of the tile
+  const int64_t N = p.N;
+  // Rolled: at most two flushes per chunk take this path, and unrolling it
+  // keeps every address live at once (twice the kernel's registers).
+#pragma unroll 1
+  for (int k = 0; k < kBlkH * TW / 64; ++k) {
+    const int e = k * 64 + lane;
+    const int r = e / TW;
+    const int col = col0 + e % TW;
+    const int64_t row = (int64_t)w * kBlkH + r;
+    if (row >= 0 && row < p.M && col < p.N) unsafeAtomicAdd(p.C + row * N + col, lds[e]);
+  }
+  wave_lds_sync();
 }
 
 // Accumulates the TC blocks [b, e) of the balanced schedule, which may span
@@ -512,8 +547,8 @@ __device__ __forceinline__ void walk_blocks(const SpmmParams &p, int b, int e, i
     // would hold up the wait for the gathers.
     const int w = wave_uniform(cur.rid);  // window of block cur.bb (lane 0)
     if (w != cw) {
-      if (cw == atomic_w0 || cw == atomic_w1) store_window<VEC, NSUB, true>(p, cw, colbase, acc);
-      else store_window<VEC, NSUB, false>(p, cw, colbase, acc);
+      if (cw == atomic_w0 || cw == atomic_w1) add_window_atomic<VEC, NSUB>(p, cw, colbase, lds, acc);
+      else store_window<VEC, NSUB>(p, cw, colbase, acc);
       zero_acc<VEC, NSUB>(acc);
       cw = w;
     }
@@ -529,8 +564,8 @@ __device__ __forceinline__ void walk_blocks(const SpmmParams &p, int b, int e, i
     cur = nxt;
     resolve_meta<true>(p, e, cur);
   }
-  if (cw == atomic_w0 || cw == atomic_w1) store_window<VEC, NSUB, true>(p, cw, colbase, acc);
-  else store_window<VEC, NSUB, false>(p, cw, colbase, acc);
+  if (cw == atomic_w0 || cw == atomic_w1) add_window_atomic<VEC, NSUB>(p, cw, colbase, lds, acc);
+  else store_window<VEC, NSUB>(p, cw, colbase, acc);
 }
 
 // End of window w whose block b is known: one ballot over the next 64 block
@@ -564,7 +599,7 @@ __device__ __forceinline__ void run_unit(const SpmmParams &p, int item, int ct, 
     const int b0 = max(0, min(p.rwo[w], b1));
     zero_acc<VEC, NSUB>(acc);
     run_blocks<PREC, VEC, NSUB>(p, b0, b1, colbase, lds, acc, has_vals);
-    store_window<VEC, NSUB, false>(p, w, colbase, acc);
+    store_window<VEC, NSUB>(p, w, colbase, acc);
   } else {
     // The chunk's run and the windows it merges atomically were worked out
     // by the preparation kernel (k_balance_prep).
@@ -586,7 +621,7 @@ __device__ __forceinline__ void run_unit(const SpmmParams &p, int item, int ct, 
 template <int PREC, int VEC, int NSUB, bool BALANCED>
 __global__ __launch_bounds__(256) void spmm_kernel(SPMM_KERNEL_ARGS) {
   const SpmmParams p = SPMM_PARAMS_FROM_ARGS;
-  __shared__ __attribute__((aligned(16))) float s_tile[kWavesPerBlock][kTileFloats];
+  __shared__ __attribute__((aligned(16))) float s_tile[kWavesPerBlock][wave_lds_floats<VEC, NSUB>()];
   const int wave = wave_uniform(threadIdx.x >> 6);
   const int ct = blockIdx.x % p.n_ct;
   const int item = (blockIdx.x / p.n_ct) * kWavesPerBlock + wave;
@@ -599,7 +634,7 @@ __global__ __launch_bounds__(256) void spmm_kernel(SPMM_KERNEL_ARGS) {
 template <int PREC, int VEC, int NSUB>
 __global__ __launch_bounds__(256) void spmm_kernel_persistent(SPMM_KERNEL_ARGS) {
   const SpmmParams p = SPMM_PARAMS_FROM_ARGS;
-  __shared__ __attribute__((aligned(16))) float s_tile[kWavesPerBlock][kTileFloats];
+  __shared__ __attribute__((aligned(16))) float s_tile[kWavesPerBlock][wave_lds_floats<VEC, NSUB>()];
   const int wave = wave_uniform(threadIdx.x >> 6);
   const int lane = lane_id();
   const int total = p.num_items * p.n_ct;
@@ -748,8 +783,12 @@ KernelFn pick_kernel(const Variant &v, int schedule) {
 }
 
 // Kernel scheme and lane shape from the options, N and operand alignment.
+// vec 0 for float32 B: 2 floats per lane (32-column tiles) in the balanced
+// schedule, 4 in the unbalanced one -- the widths measured fastest for each
+// across the selector's 2,000 training matrices -- or the widest narrower
+// width N and the alignment allow.
 dtc_status_t resolve_shape(const dtc_spmm_options_t *opt, int32_t n, const void *b, const float *c,
-                           Variant *out) {
+                           bool balanced, Variant *out) {
   const int prec = opt ? opt->precision : DTC_PRECISION_FP32;
   const int bdt = opt ? opt->b_dtype : DTC_B_FLOAT32;
   int vec = opt ? opt->vec : 0;
@@ -761,7 +800,9 @@ dtc_status_t resolve_shape(const dtc_spmm_options_t *opt, int32_t n, const void 
     auto fits = [&](int v) {
       return n % v == 0 && ((uintptr_t)b % (2 * v)) == 0 && ((uintptr_t)c % (4 * v)) == 0;
     };
-    if (vec == 0) vec = 4;  // 64-column tiles: measured faster than 128 (more waves in flight)
+    // Balanced: 128-column tiles, faster than 64 on R-MAT scales 18 and 20 (0.44 vs
+    // 0.49 ms and 1.89 vs 2.09 ms at chunk 64, N = 256); unbalanced: 64.
+    if (vec == 0) vec = (balanced && fits(8)) ? 8 : 4;
     if (!fits(vec))
       return fail(DTC_ERROR_INVALID_VALUE,
                   "bfloat16 B needs N to be a multiple of 4 and 8-byte aligned B/C rows");
@@ -777,7 +818,10 @@ dtc_status_t resolve_shape(const dtc_spmm_options_t *opt, int32_t n, const void 
   auto fits = [&](int v) {
     return n % v == 0 && ((uintptr_t)b % (4 * v)) == 0 && ((uintptr_t)c % (4 * v)) == 0;
   };
-  if (vec == 0) vec = fits(4) ? 4 : (fits(2) ? 2 : 1);
+  if (vec == 0) {
+    const int pref = balanced ? 2 : 4;
+    vec = (pref >= 4 && fits(4)) ? 4 : (fits(2) ? 2 : 1);
+  }
   if (!fits(vec))
     return fail(DTC_ERROR_INVALID_VALUE,
                 "vec " + std::to_string(vec) + " needs N and the B/C base addresses to be multiples of it");
@@ -811,6 +855,17 @@ dtc_status_t check_common(const uint8_t *tile, const int32_t *off, const int32_t
   if ((int64_t)num_nodes * n > ((int64_t)1 << 40))
     return fail(DTC_ERROR_UNSUPPORTED, "output too large");
   return DTC_SUCCESS;
+}
+
+// Default balanced chunk: the largest power of two in [32, 512] that is at
+// most T / 8192, i.e. about 8K chunks (x column tiles) of work units once
+// the matrix is large enough.  On the selector's 400 held-out matrices this
+// rule is 10% slower than the per-matrix best variant on average, a fixed
+// 256 121% (small matrices leave most CUs idle with long chunks).
+int default_chunk_blocks(int32_t num_blocks) {
+  int c = 32;
+  while (c < 512 && (int64_t)c * 2 * 8192 <= (int64_t)num_blocks) c *= 2;
+  return c;
 }
 
 // Balanced workspace: the persistent work counter, then one int2 run per
@@ -904,7 +959,7 @@ __global__ __launch_bounds__(256) void spmm_tcgnn_kernel(
     gather_b<DTC_PRECISION_FP32, VEC, 1>(p, rows, colbase, breg);
     mfma_group<DTC_PRECISION_FP32, VEC, 1>(a, breg, acc, true);
   }
-  store_window<VEC, 1, false>(p, w, colbase, acc);
+  store_window<VEC, 1>(p, w, colbase, acc);
 }
 
 }  // namespace dtc
@@ -936,7 +991,7 @@ dtc_status_t dtc_spmm_tcgnn(const int32_t *node_pointer, const int32_t *edge_lis
   o.vec = options ? options->vec : 0;
   o.subtiles = 1;
   Variant var;
-  dtc_status_t st = resolve_shape(&o, n, b, c, &var);
+  dtc_status_t st = resolve_shape(&o, n, b, c, false, &var);
   if (st != DTC_SUCCESS) return st;
   const int32_t W = (int32_t)ceil_div(num_nodes, kBlkH);
   if (b_rows == 0) return zero_output(c, num_nodes, n, stream);
@@ -987,7 +1042,7 @@ dtc_status_t dtc_spmm_windows(const int32_t *row_window_offset, const uint8_t *t
     return fail(DTC_ERROR_INVALID_VALUE,
                 "Rowwindow_offset has " + std::to_string(num_windows) + " windows, fewer than num_nodes/16");
   Variant var;
-  st = resolve_shape(options, n, b, c, &var);
+  st = resolve_shape(options, n, b, c, false, &var);
   if (st != DTC_SUCCESS) return st;
   // Only windows that hold output rows are processed.
   const int32_t W = (int32_t)std::min<int64_t>(num_windows, ceil_div(num_nodes, kBlkH));
@@ -1040,9 +1095,10 @@ dtc_status_t dtc_spmm_balanced(const int32_t *tcblock_rowid, const uint8_t *tcbl
   if (num_blocks > 0 && (!tcblock_rowid || !tcblocktile_id || !sparse_atox))
     return fail(DTC_ERROR_INVALID_VALUE, "TCblock_rowid, TCblocktile_id and sparse_AToX_idx are required");
   Variant var;
-  st = resolve_shape(options, n, b, c, &var);
+  st = resolve_shape(options, n, b, c, true, &var);
   if (st != DTC_SUCCESS) return st;
-  const int chunk = (options && options->chunk_blocks > 0) ? options->chunk_blocks : 256;
+  const int chunk = (options && options->chunk_blocks > 0) ? options->chunk_blocks
+                                                           : default_chunk_blocks(num_blocks);
   const bool persistent = options && options->persistent;
   const int32_t W = (int32_t)ceil_div(num_nodes, kBlkH);
   if (W == 0) return DTC_SUCCESS;

@@ -55,9 +55,22 @@ VARIANTS = [
     ("balanced", "bf16x6", 256, 2, 1, False),
     ("balanced", "bf16x6", 256, 4, 1, False),
 ]
-# The method without the selector: the library's default options (balanced,
-# FP32, 256-block chunks, 64-column tiles).
-DEFAULT_VARIANT = 11
+
+
+def default_chunk_blocks(num_blocks: int) -> int:
+    """The library's balanced chunk when options leave it at 0: the largest
+    power of two in [32, 512] at most num_blocks / 8192 (csrc/spmm.hip)."""
+    c = 32
+    while c < 512 and c * 2 * 8192 <= num_blocks:
+        c *= 2
+    return c
+
+
+def default_variant(num_blocks: int) -> int:
+    """Index in VARIANTS of what the library runs with default options (the
+    method without the selector): balanced, FP32, 32-column tiles, the
+    default chunk."""
+    return VARIANTS.index(("balanced", "fp32", default_chunk_blocks(int(num_blocks)), 2, 1, False))
 
 FEATURE_NAMES = [
     "log2_rows", "log2_nnz", "log2_N", "log2_avg_deg", "deg_cv", "log2_max_deg",
@@ -362,15 +375,16 @@ def best_fixed_variant(times_ms) -> int:
 
 
 def evaluate(model: MLPSelector, data, holdout_from: int = 1600) -> dict:
-    """Held-out regret of the selector, of the fixed default variant and of
-    the best single variant chosen on the training split (the strongest
-    method without a selector)."""
+    """Held-out regret of the selector, of the library's default options
+    (their variant depends on the block count) and of the best single
+    variant chosen on the training split (the strongest fixed method)."""
     train, test = split(data, holdout_from)
     X, t = data["features"][test], data["times_ms"][test]
     chosen = model.choose(X)
     fixed = best_fixed_variant(data["times_ms"][train])
+    dflt = np.array([default_variant(b) for b in data["blocks"][test]])
     r_sel = regret(t, chosen)
-    r_def = regret(t, np.full(t.shape[0], DEFAULT_VARIANT))
+    r_def = regret(t, dflt)
     r_fix = regret(t, np.full(t.shape[0], fixed))
     picked = t[np.arange(t.shape[0]), chosen]
     fallback = np.array([bool(model.out_of_range(x)) for x in X])
@@ -379,11 +393,11 @@ def evaluate(model: MLPSelector, data, holdout_from: int = 1600) -> dict:
             "fallback_to_exhaustive": int(fallback.sum()),
             "guarded_mean_regret": float(r_guard.mean()),
             "selector_mean_regret": float(r_sel.mean()), "selector_p90_regret": float(np.quantile(r_sel, 0.9)),
-            "default_variant": variant_name(VARIANTS[DEFAULT_VARIANT]),
+            "default_variant": "balanced:fp32:c<T/8192 in 32..512>:v2x1",
             "default_mean_regret": float(r_def.mean()), "default_p90_regret": float(np.quantile(r_def, 0.9)),
             "best_fixed_variant": variant_name(VARIANTS[fixed]),
             "best_fixed_mean_regret": float(r_fix.mean()), "best_fixed_p90_regret": float(np.quantile(r_fix, 0.9)),
-            "mean_speedup_over_default": float(np.exp(np.log(t[:, DEFAULT_VARIANT] / picked).mean())),
+            "mean_speedup_over_default": float(np.exp(np.log(t[np.arange(t.shape[0]), dflt] / picked).mean())),
             "mean_speedup_over_best_fixed": float(np.exp(np.log(t[:, fixed] / picked).mean())),
             "exact_pick_rate": float((r_sel < 1e-9).mean())}
 

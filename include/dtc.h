@@ -52,10 +52,13 @@ typedef enum dtc_b_dtype {
 typedef struct dtc_spmm_options {
   int32_t precision;    /* dtc_precision_t */
   int32_t vec;          /* B elements per lane per gathered row: 1, 2, 4 (float32 B);
-                           4, 8 (bfloat16 B); 0 = the widest float32 width N and
-                           alignment allow, 4 for bfloat16 B */
+                           4, 8 (bfloat16 B); 0 = float32 B: 2 (balanced) or 4
+                           (unbalanced), narrower where N or the alignment need it;
+                           bfloat16 B: 8 (balanced; 4 where N or the alignment need
+                           it) or 4 (unbalanced) */
   int32_t subtiles;     /* 1 or 2 column tiles of 16*vec per wave (0 = 1; float32 B) */
-  int32_t chunk_blocks; /* balanced schedule: TC blocks per work chunk (0 = 256) */
+  int32_t chunk_blocks; /* balanced schedule: TC blocks per work chunk (0 = the largest
+                           power of two in [32, 512] at most num_blocks / 8192) */
   int32_t persistent;   /* balanced schedule: 1 = persistent waves pulling chunks from a queue */
   int32_t b_dtype;      /* dtc_b_dtype_t */
 } dtc_spmm_options_t;
