@@ -666,6 +666,44 @@ __device__ __forceinline__ void zero_window_rows(float *C, int64_t w, int32_t M,
 }
 
 constexpr int kPrepWindowsPerWave = 8;
+constexpr int kNearSpan = 512;  // chunks up to this many blocks use the bounded scans
+
+// Start of window w (the window of block c) if it lies at most `span` blocks
+// before c, else -1.  All span/64 loads per lane are issued before the first
+// ballot: one memory round trip instead of a search's chain.
+__device__ __forceinline__ int window_start_near(const int32_t *rowid, int c, int w, int span) {
+  const int lane = lane_id();
+  int v[kNearSpan / 64];
+#pragma unroll
+  for (int k = 0; k < kNearSpan / 64; ++k) {
+    const int idx = c - 1 - lane - 64 * k;  // c-1, c-2, ...; -2 marks the array's start
+    v[k] = (64 * k < span && idx >= c - span) ? (idx >= 0 ? rowid[idx] : -2) : w;
+  }
+#pragma unroll
+  for (int k = 0; k < kNearSpan / 64; ++k) {
+    const uint64_t m = __ballot(v[k] != w);
+    if (m) return c - 64 * k - (int)__builtin_ctzll(m);
+  }
+  return -1;
+}
+
+// End of window w (the window of block c) if it lies at most `span` blocks
+// after c, else -1 (same single round of loads).
+__device__ __forceinline__ int window_end_near(const int32_t *rowid, int c, int T, int w, int span) {
+  const int lane = lane_id();
+  int v[kNearSpan / 64 + 1];  // blocks c .. c + span
+#pragma unroll
+  for (int k = 0; k < kNearSpan / 64 + 1; ++k) {
+    const int idx = c + lane + 64 * k;  // c, c+1, ...; -2 marks the array's end
+    v[k] = (64 * k <= span && idx <= c + span) ? (idx < T ? rowid[idx] : -2) : w;
+  }
+#pragma unroll
+  for (int k = 0; k < kNearSpan / 64 + 1; ++k) {
+    const uint64_t m = __ballot(v[k] != w);
+    if (m) return c + 64 * k + (int)__builtin_ctzll(m);
+  }
+  return -1;
+}
 
 // Balanced-schedule preparation.  Ownership rule: a window of at most `chunk`
 // blocks belongs whole to the chunk holding its first block (and may run
@@ -711,22 +749,33 @@ __global__ __launch_bounds__(256) void k_balance_prep(const int32_t *rowid, int3
   }
   const int c0 = (int)v * chunk;
   const int c1 = min(c0 + chunk, T);
+  // Window extents: exact for chunks above kNearSpan (searches); for smaller
+  // chunks only within `chunk` blocks of the boundary, -1 meaning farther --
+  // which already makes the window longer than a chunk.
+  const bool near = chunk <= kNearSpan;
+  auto start_of = [&](int c, int w) {
+    return wave_uniform(near ? window_start_near(rowid, c, w, chunk) : window_start_from(rowid, c, w));
+  };
+  auto end_of = [&](int c, int w) {
+    return wave_uniform(near ? window_end_near(rowid, c, T, w, chunk)
+                             : max(c + 1, window_end_from(rowid, c, T, w)));
+  };
   const int w0 = wave_uniform(rowid[c0]);
-  const int ws0 = wave_uniform(window_start_from(rowid, c0, w0));
-  const int we0 = wave_uniform(max(c0 + 1, window_end_from(rowid, c0, T, w0)));
-  const bool first_long = we0 - ws0 > chunk;
+  const int ws0 = start_of(c0, w0);
+  const int we0 = end_of(c0, w0);
+  const bool first_long = ws0 < 0 || we0 < 0 || we0 - ws0 > chunk;
   int b = (!first_long && ws0 < c0) ? we0 : c0;
   int e = c1;
   bool last_long = false;
-  int wl = -1, wsl = 0;
+  int wl = -1, wsl = -1;
   if (c1 < T) {
     const int r_in = wave_uniform(rowid[c1 - 1]);
     const int r_out = wave_uniform(rowid[c1]);
     if (r_in == r_out) {  // a window crosses c1
       wl = r_in;
-      wsl = r_in == w0 ? ws0 : wave_uniform(window_start_from(rowid, c1, r_in));
-      const int wel = r_in == w0 ? we0 : wave_uniform(max(c1 + 1, window_end_from(rowid, c1, T, r_in)));
-      if (wel - wsl > chunk) last_long = true;
+      wsl = r_in == w0 ? ws0 : start_of(c1, r_in);
+      const int wel = r_in == w0 ? we0 : end_of(c1, r_in);
+      if (wsl < 0 || wel < 0 || wel - wsl > chunk) last_long = true;
       else e = wel;
     }
   }
