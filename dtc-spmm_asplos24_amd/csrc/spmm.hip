@@ -485,6 +485,31 @@ __device__ __forceinline__ void store_window(const SpmmParams &p, int w, int col
   }
 }
 
+// Zeroes this wave's column tile of windows [w0, w1) (empty windows: no TC
+// block writes them, so any wave may, at any time of the launch).
+template <int VEC, int NSUB>
+__device__ __forceinline__ void zero_windows(const SpmmParams &p, int w0, int w1, int colbase) {
+  typedef typename VecType<VEC>::type vfloat;
+  const vfloat z = vzero<VEC>();
+  const int g = lane_id() >> 4;
+  const int64_t N = p.N;
+  // Rolled: rare, and unrolled stores would hold their addresses in registers
+  // across the whole SpMM loop.
+#pragma unroll 1
+  for (int r = max(w0, 0) * kBlkH; r < w1 * kBlkH; r += kBlkH) {
+#pragma unroll 1
+    for (int i = 0; i < 4; ++i) {
+      const int64_t row = (int64_t)r + 4 * g + i;
+      if (row >= p.M) continue;
+#pragma unroll
+      for (int s = 0; s < NSUB; ++s) {
+        const int col = colbase + s * 16 * VEC;
+        if (col < p.N) __builtin_nontemporal_store(z, reinterpret_cast<vfloat *>(p.C + row * N + col));
+      }
+    }
+  }
+}
+
 // Adds the 16 rows of window w held in acc to C with atomics, through this
 // wave's LDS: the tile is written in the accumulator layout and read back
 // row-major, so that each atomic instruction covers 64 consecutive floats of
@@ -525,7 +550,10 @@ __device__ __forceinline__ void add_window_atomic(const SpmmParams &p, int w, in
 // a new window the accumulators of the previous one are stored and cleared.
 // Window `atomic_w0` (if >= 0) and `atomic_w1` (if >= 0) are merged with
 // atomic adds (windows cut at chunk boundaries), every other window is
-// stored whole.  Restarting the pipeline per window instead costs one
+// stored whole.  Empty windows are zeroed by the wave that passes them: the
+// gap before the run's first window (after the previous block's window), the
+// gaps between consecutive windows, and -- for the run that ends at the last
+// block -- the windows after it.  Restarting the pipeline per window instead costs one
 // metadata round trip per window, and most windows of skewed graphs hold
 // only a few blocks.
 template <int PREC, int VEC, int NSUB>
@@ -534,8 +562,12 @@ __device__ __forceinline__ void walk_blocks(const SpmmParams &p, int b, int e, i
                                             bool has_vals, int atomic_w0, int atomic_w1) {
   GroupMeta cur;
   load_meta<true>(p, b, e, has_vals, cur);
+  const int prev_w = b > 0 ? wave_uniform(p.rowid[b - 1]) : -1;
   resolve_meta<true>(p, e, cur);
   int cw = wave_uniform(cur.rid);
+  // pending empty windows [gap0, gap1): zeroed at one place in the loop
+  int gap0 = prev_w + 1;
+  int gap1 = min(cw, p.num_windows);
   zero_acc<VEC, NSUB>(acc);
   while (true) {
     breg_t<PREC, VEC> breg[NSUB][8];
@@ -550,7 +582,13 @@ __device__ __forceinline__ void walk_blocks(const SpmmParams &p, int b, int e, i
       if (cw == atomic_w0 || cw == atomic_w1) add_window_atomic<VEC, NSUB>(p, cw, colbase, lds, acc);
       else store_window<VEC, NSUB>(p, cw, colbase, acc);
       zero_acc<VEC, NSUB>(acc);
+      gap0 = cw + 1;
+      gap1 = min(w, p.num_windows);
       cw = w;
+    }
+    if (gap0 < gap1) {
+      zero_windows<VEC, NSUB>(p, gap0, gap1, colbase);
+      gap0 = gap1;
     }
     const int bn = cur.bb + cur.nblk;
     const bool more = bn < e;
@@ -566,6 +604,7 @@ __device__ __forceinline__ void walk_blocks(const SpmmParams &p, int b, int e, i
   }
   if (cw == atomic_w0 || cw == atomic_w1) add_window_atomic<VEC, NSUB>(p, cw, colbase, lds, acc);
   else store_window<VEC, NSUB>(p, cw, colbase, acc);
+  if (e == p.num_blocks) zero_windows<VEC, NSUB>(p, cw + 1, p.num_windows, colbase);
 }
 
 // End of window w whose block b is known: one ballot over the next 64 block
@@ -665,7 +704,6 @@ __device__ __forceinline__ void zero_window_rows(float *C, int64_t w, int32_t M,
   }
 }
 
-constexpr int kPrepWindowsPerWave = 8;
 constexpr int kNearSpan = 512;  // chunks up to this many blocks use the bounded scans
 
 // Start of window w (the window of block c) if it lies at most `span` blocks
@@ -708,12 +746,7 @@ __device__ __forceinline__ int window_end_near(const int32_t *rowid, int c, int 
 // Balanced-schedule preparation.  Ownership rule: a window of at most `chunk`
 // blocks belongs whole to the chunk holding its first block (and may run
 // past that chunk's end); longer windows are cut at chunk boundaries and
-// their pieces merged with atomic adds.  Waves v < num_items take chunk v,
-// the following waves 8 windows each, so that neither kind waits for the
-// other's searches.  Wave v = num_items + u
-//   * zeroes the empty windows among [8u, 8u + 8) (a window id absent from
-//     rowid, found by one search and then one look-ahead per window).
-// Wave v < num_items
+// their pieces merged with atomic adds.  Wave v, one per chunk:
 //   * for chunk v ([c0, c1) = blocks [v*chunk, (v+1)*chunk)), writes the
 //     chunk's run [b, e) to runs[v]: b = c0 unless c0 lies inside a short
 //     window that began earlier (then b = that window's end), e = c1 unless
@@ -723,30 +756,16 @@ __device__ __forceinline__ int window_end_near(const int32_t *rowid, int c, int 
 //     chunk, since no chunk stores them whole;
 //   * (wave 0) resets the persistent work counter.
 // The window searches run here, once per chunk, on an idle memory system
-// instead of at the start of every SpMM wave.
+// instead of at the start of every SpMM wave.  Empty windows are zeroed by
+// the SpMM waves themselves (walk_blocks): nothing else writes them.
 __global__ __launch_bounds__(256) void k_balance_prep(const int32_t *rowid, int32_t T,
-                                                      int32_t W, int32_t M, int32_t N,
+                                                      int32_t M, int32_t N,
                                                       int32_t chunk, int32_t num_items,
                                                       float *C, int32_t *queue, int2 *runs) {
   if (queue && blockIdx.x == 0 && threadIdx.x == 0) *queue = 0;
   const int lane = lane_id();
   const int64_t v = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-  if (v >= num_items) {
-    const int64_t wa = (v - num_items) * kPrepWindowsPerWave;
-    if (wa >= W) return;
-    const int wb = (int)(wa + kPrepWindowsPerWave < W ? wa + kPrepWindowsPerWave : W);
-    int lb = wave_uniform(wave_lower_bound<int32_t>(rowid, 0, T, (int32_t)wa));
-    for (int w = (int)wa; w < wb; ++w) {
-      // nx = first block with id > w; blocks of window w are [lb, nx)
-      const int idx = lb + lane;
-      const uint64_t m = __ballot(idx >= T || rowid[idx] > w);
-      const int nx = wave_uniform(m ? lb + (int)__builtin_ctzll(m)
-                                    : wave_lower_bound<int32_t>(rowid, lb + 64, T, w + 1));
-      if (nx == lb) zero_window_rows(C, w, M, N);
-      lb = nx;
-    }
-    return;
-  }
+  if (v >= num_items) return;
   const int c0 = (int)v * chunk;
   const int c1 = min(c0 + chunk, T);
   // Window extents: exact for chunks above kNearSpan (searches); for smaller
@@ -1175,9 +1194,9 @@ dtc_status_t dtc_spmm_balanced(const int32_t *tcblock_rowid, const uint8_t *tcbl
   p.queue = persistent ? static_cast<int32_t *>(workspace) : nullptr;
   int2 *runs = reinterpret_cast<int2 *>(static_cast<char *>(workspace) + kQueueBytes);
   p.runs = runs;
-  const int64_t prep_waves = (int64_t)p.num_items + ceil_div(W, kPrepWindowsPerWave);
+  const int64_t prep_waves = p.num_items;
   k_balance_prep<<<(unsigned)ceil_div(prep_waves, kWavesPerBlock), 256, 0, stream>>>(
-      tcblock_rowid, num_blocks, W, num_nodes, n, chunk, p.num_items, c, p.queue, runs);
+      tcblock_rowid, num_blocks, num_nodes, n, chunk, p.num_items, c, p.queue, runs);
   DTC_LAUNCH_CHECK("balance_prep");
   if (!persistent) {
     KernelFn fn = pick_kernel(var, 1);
