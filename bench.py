@@ -52,6 +52,22 @@ def timed(stream, fn, iters):
     return e0.elapsed_time(e1) / iters
 
 
+def timed_steps(stream, fn, iters):
+    """(mean, median) ms per step: one event before every step and one after
+    the last, all on the launch stream; the mean is first to last event."""
+    import torch
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(iters + 1)]
+    ev[0].record(stream)
+    for i in range(iters):
+        fn()
+        ev[i + 1].record(stream)
+    ev[-1].synchronize()
+    steps = sorted(ev[i].elapsed_time(ev[i + 1]) for i in range(iters))
+    mid = len(steps) // 2
+    median = steps[mid] if len(steps) % 2 else 0.5 * (steps[mid - 1] + steps[mid])
+    return ev[0].elapsed_time(ev[-1]) / iters, median
+
+
 def copy_probe():
     """Three copies of a 1 GiB float32 tensor: each reads and writes exactly
     2**30 bytes, the reference point for FETCH_SIZE / WRITE_SIZE."""
@@ -386,7 +402,7 @@ def main():
     for _ in range(args.warmup):
         fn()
     torch.cuda.synchronize()
-    ms = timed(stream, fn, args.steps)
+    ms, ms_median = timed_steps(stream, fn, args.steps)
     torch.cuda.synchronize()
     comp_err, norm_err = errors(C)
 
@@ -505,6 +521,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms, 4),
+        "ms_median_step": round(ms_median, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
