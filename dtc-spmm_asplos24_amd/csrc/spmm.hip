@@ -928,8 +928,8 @@ dtc_status_t check_common(const uint8_t *tile, const int32_t *off, const int32_t
 // Default balanced chunk: the largest power of two in [32, 512] that is at
 // most T / 8192, i.e. about 8K chunks (x column tiles) of work units once
 // the matrix is large enough.  On the selector's 400 held-out matrices this
-// rule is 10% slower than the per-matrix best variant on average, a fixed
-// 256 121% (small matrices leave most CUs idle with long chunks).
+// rule is 12% slower than the per-matrix best variant on average, a fixed
+// 256 122% (small matrices leave most CUs idle with long chunks).
 int default_chunk_blocks(int32_t num_blocks) {
   int c = 32;
   while (c < 512 && (int64_t)c * 2 * 8192 <= (int64_t)num_blocks) c *= 2;

@@ -61,7 +61,9 @@ def test_dataset_and_model_consistent():
     if os.path.exists(DEFAULT_MODEL):
         m = np.load(DEFAULT_MODEL)
         assert [str(s) for s in m["variants"]] == [variant_name(v) for v in VARIANTS]
-        assert m["w0"].shape[0] == len(FEATURE_NAMES) and m["w2"].shape[1] == len(VARIANTS)
+        assert int(m["members"]) >= 1
+        for k in range(int(m["members"])):
+            assert m[f"m{k}_w0"].shape[0] == len(FEATURE_NAMES) and m[f"m{k}_w2"].shape[1] == len(VARIANTS)
 
 
 def test_model_range_guard():
