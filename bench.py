@@ -510,6 +510,15 @@ def main():
         except Exception as exc:  # reported, never fatal to the headline measurement
             reordered = {"error": f"{type(exc).__name__}: {exc}"}
             log(f"reordered run failed: {exc}")
+        # what the reorder advisor (dtc/reorder_advice.py) predicted from the
+        # original matrix's features, beside the measured speedup
+        try:
+            from dtc.reorder_advice import THRESHOLD, ReorderAdvisor
+            pred = float(ReorderAdvisor.load().predict_gain(feats[None, :])[0])
+            reordered["advisor"] = {"predicted_speedup": round(float(np.exp(pred)), 3),
+                                    "advises_reorder": bool(pred > THRESHOLD)}
+        except Exception as exc:
+            reordered["advisor"] = {"error": f"{type(exc).__name__}: {exc}"}
 
     dtypes = {"fp32": "f32", "bf16x6": "bf16x6 (f32 accumulate)", "bf16x3": "bf16x3 (f32 accumulate)",
               "fp16": "f16 (f32 accumulate)", "bf16": "bf16 (f32 accumulate)"}
