@@ -1,12 +1,14 @@
 // ME-TCF SpMM on gfx950: C = A * B with A in 16 x 8 TC blocks.
 //
 // Work decomposition (one wavefront = one work unit, 4 waves per workgroup):
-//   unit = (run of TC blocks of ONE row window, column tile of 16*VEC*NSUB
-//   output columns).  The unbalanced schedule uses a whole window per unit
+//   unit = (run of TC blocks, column tile of 16*VEC*NSUB output columns).
+//   The unbalanced schedule uses one whole window per unit
 //   (DTCSpMM_kernel.cu:811-944 launches one CTA per window); the balanced
-//   schedule cuts the TC-block list into fixed chunks and splits every chunk
-//   at window boundaries (DTCSpMM_kernel.cu:4730-4901 walks 64-block chunks
-//   and flushes on a window change).
+//   schedule cuts the TC-block list into fixed chunks, each unit walking the
+//   windows that start in its chunk and flushing on every window change, with
+//   windows longer than a chunk cut and merged atomically
+//   (DTCSpMM_kernel.cu:4730-4901 walks 64-block chunks and flushes on a
+//   window change).
 //
 // Inner loop: 4 consecutive TC blocks of the window form one K = 32 group,
 // which is exactly the K of v_mfma_f32_16x16x32_{bf16,f16}.  Lane l works on
@@ -553,9 +555,8 @@ __device__ __forceinline__ void add_window_atomic(const SpmmParams &p, int w, in
 // stored whole.  Empty windows are zeroed by the wave that passes them: the
 // gap before the run's first window (after the previous block's window), the
 // gaps between consecutive windows, and -- for the run that ends at the last
-// block -- the windows after it.  Restarting the pipeline per window instead costs one
-// metadata round trip per window, and most windows of skewed graphs hold
-// only a few blocks.
+// block -- the windows after it.  Window starts come from the look-ahead of
+// window ids loaded with each group's metadata, so no window is searched for.
 template <int PREC, int VEC, int NSUB>
 __device__ __forceinline__ void walk_blocks(const SpmmParams &p, int b, int e, int colbase,
                                             float *lds, floatx4 (&acc)[NSUB][VEC],
