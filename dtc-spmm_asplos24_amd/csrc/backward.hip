@@ -59,22 +59,16 @@ __global__ __launch_bounds__(256) void k_check_csr(const int32_t *rowptr, const 
   if (f) atomicOr(flags, f);
 }
 
+// Row, key (the column) and value (the CSR index) of every edge, edge-balanced.
 __global__ __launch_bounds__(256) void k_transpose_keys(const int32_t *rowptr,
                                                         const int32_t *col, int32_t M,
-                                                        int32_t *e2r, uint64_t *keys,
-                                                        int32_t *vals) {
-  const int lane = threadIdx.x & 63;
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  for (int64_t r = wave; r < M; r += nwaves) {
-    const int s = rowptr[r];
-    const int e = rowptr[r + 1];
-    for (int idx = s + lane; idx < e; idx += 64) {
-      e2r[idx] = (int32_t)r;
-      keys[idx] = (uint32_t)col[idx];
-      vals[idx] = idx;
-    }
-  }
+                                                        int32_t nnz, int32_t *e2r,
+                                                        uint64_t *keys, int32_t *vals) {
+  for_each_edge_row(rowptr, M, nnz, [&](int idx, int row) {
+    e2r[idx] = row;
+    keys[idx] = (uint32_t)col[idx];
+    vals[idx] = idx;
+  });
 }
 
 __global__ __launch_bounds__(256) void k_transpose_fill(const uint64_t *skeys,
@@ -260,8 +254,8 @@ dtc_status_t dtc_csr_transpose(const int32_t *node_pointer, const int32_t *edge_
   if (flags)
     return fail(DTC_ERROR_INVALID_CSR, "invalid CSR for transpose (flags " + std::to_string(flags) + ")");
   if (nnz > 0) {
-    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(65536, ceil_div(num_rows, 4)));
-    k_transpose_keys<<<grid, 256, 0, stream>>>(node_pointer, edge_list, num_rows, ws.e2r,
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(65536, ceil_div(nnz, 4096)));
+    k_transpose_keys<<<grid, 256, 0, stream>>>(node_pointer, edge_list, num_rows, nnz, ws.e2r,
                                                ws.keys0, ws.vals0);
     DTC_LAUNCH_CHECK("transpose_keys");
   }

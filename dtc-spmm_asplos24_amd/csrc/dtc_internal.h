@@ -146,6 +146,48 @@ __device__ __forceinline__ int wave_lower_bound(const T *a, int lo, int hi, T ke
   return lo + __popcll(__ballot(less));
 }
 
+// Calls emit(idx, row) once for every edge idx of a CSR with M rows, the
+// edges cut into equal ranges over all waves of the grid (a long row is
+// shared by many waves instead of walked by one).  The wave finds the row of
+// its first edge by one search, then takes up to 64 edges per step, each
+// lane locating its edge's row among the next 64 row ends held in lanes.
+// Must be called by whole waves with the same arguments.
+template <class Emit>
+__device__ __forceinline__ void for_each_edge_row(const int32_t *rowptr, int32_t M, int32_t nnz,
+                                                  Emit emit) {
+  const int lane = lane_id();
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+  const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave_uniform(threadIdx.x >> 6);
+  const int e_begin = (int)((int64_t)nnz * w / nw);
+  const int e_end = (int)((int64_t)nnz * (w + 1) / nw);
+  if (e_begin >= e_end) return;
+  // the row of e_begin: the last r with rowptr[r] <= e_begin
+  int r = wave_uniform(wave_lower_bound<int32_t>(rowptr, 0, M + 1, e_begin + 1) - 1);
+  int p = e_begin;
+  while (p < e_end) {
+    const int64_t bi = (int64_t)r + 1 + lane;
+    const int bnd = bi <= M ? rowptr[bi] : INT32_MAX;
+    const uint64_t past = __ballot(bnd > p);
+    if (past == 0) {  // rows r .. r+63 all end at or before p (empty rows)
+      r += kWave;
+      continue;
+    }
+    const int last = __builtin_amdgcn_readlane(bnd, kWave - 1);  // > p
+    const int cnt = min(min(kWave, e_end - p), last - p);
+    const int idx = p + min(lane, cnt - 1);
+    int lo = 0, hi = kWave - 1;  // first lane whose row end exceeds idx (lane 63's does)
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (__shfl(bnd, mid, kWave) > idx) hi = mid;
+      else lo = mid + 1;
+    }
+    const int row = r + lo;
+    if (lane < cnt) emit(idx, row);
+    r = wave_uniform(__shfl(row, cnt - 1, kWave));  // row of the step's last edge
+    p += cnt;
+  }
+}
+
 // ----------------------------------------------------- device-wide pieces
 // (implemented in primitives.hip)
 

@@ -148,41 +148,11 @@ __global__ __launch_bounds__(256) void k_rows_keys(const int32_t *rowptr, const 
                                                    int32_t M, int32_t nnz, int32_t blk_h,
                                                    int colbits, int32_t *e2r, uint64_t *keys,
                                                    int32_t *vals) {
-  const int lane = lane_id();
-  const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
-  const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave_uniform(threadIdx.x >> 6);
-  const int e_begin = (int)((int64_t)nnz * w / nw);
-  const int e_end = (int)((int64_t)nnz * (w + 1) / nw);
-  if (e_begin >= e_end) return;
-  // the row of e_begin: the last r with rowptr[r] <= e_begin
-  int r = wave_uniform(wave_lower_bound<int32_t>(rowptr, 0, M + 1, e_begin + 1) - 1);
-  int p = e_begin;
-  while (p < e_end) {
-    const int64_t bi = (int64_t)r + 1 + lane;
-    const int bnd = bi <= M ? rowptr[bi] : INT32_MAX;
-    const uint64_t past = __ballot(bnd > p);
-    if (past == 0) {  // rows r .. r+63 all end at or before p (empty rows)
-      r += kWave;
-      continue;
-    }
-    const int last = __builtin_amdgcn_readlane(bnd, kWave - 1);  // > p
-    const int cnt = min(min(kWave, e_end - p), last - p);
-    const int idx = p + min(lane, cnt - 1);
-    int lo = 0, hi = kWave - 1;  // first lane whose row end exceeds idx (lane 63's does)
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (__shfl(bnd, mid, kWave) > idx) hi = mid;
-      else lo = mid + 1;
-    }
-    const int row = r + lo;
-    if (lane < cnt) {
-      e2r[idx] = row;
-      keys[idx] = ((uint64_t)(row / blk_h) << colbits) | (uint32_t)col[idx];
-      vals[idx] = idx;
-    }
-    r = wave_uniform(__shfl(row, cnt - 1, kWave));  // row of the step's last edge
-    p += cnt;
-  }
+  for_each_edge_row(rowptr, M, nnz, [&](int idx, int row) {
+    e2r[idx] = row;
+    keys[idx] = ((uint64_t)(row / blk_h) << colbits) | (uint32_t)col[idx];
+    vals[idx] = idx;
+  });
 }
 
 __device__ __forceinline__ int window_begin(const int32_t *rowptr, int w, int blk_h, int M) {

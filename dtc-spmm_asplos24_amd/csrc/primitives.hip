@@ -138,7 +138,7 @@ dtc_status_t scan_key_changes(const uint64_t *keys, int32_t *out, int64_t n, voi
 // -------------------------------------------------------------- radix sort
 
 constexpr int kSortThreads = 256;
-constexpr int kSortItems = 16;
+constexpr int kSortItems = 8;
 constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr int kRadix = 256;
 
@@ -163,18 +163,54 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint64_t
 // Stable scatter: element i of the tile goes to offsets[d][tile] + (number of
 // earlier elements of the tile with the same digit d).  Elements are taken in
 // rounds of 256 in index order; inside a round the rank among equal digits is
-// found with eight ballots (one per digit bit) and a popcount.
+// found with eight ballots (one per digit bit) and a popcount.  The tile is
+// first laid out in LDS in digit order (its digit counts are differences of
+// consecutive scanned offsets), then written out so that consecutive threads
+// store consecutive elements of one digit's run: runs of ~16 keys land in
+// consecutive global addresses instead of 4,096 scattered 12-byte writes.
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const uint64_t *kin, const int32_t *vin, uint64_t *kout, int32_t *vout, int64_t n,
     int shift, const int32_t *offsets, int ntiles) {
-  __shared__ int s_off[kRadix];
+  __shared__ int s_off[kRadix];   // global position of the tile's first element of digit d
+  __shared__ int s_loc[kRadix];   // local position of the same element
+  __shared__ int s_run[kRadix];   // elements of digit d placed so far
   __shared__ int s_wcnt[4][kRadix];
+  __shared__ uint64_t s_keys[kSortTile];
+  __shared__ int32_t s_vals[kSortTile];
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
   const int lane = tid & 63;
   const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-  s_off[tid] = offsets[(int64_t)tid * ntiles + blockIdx.x];
+  const int64_t idx = (int64_t)tid * ntiles + blockIdx.x;
+  const int64_t total = (int64_t)kRadix * ntiles;
+  const int off = offsets[idx];
+  const int cnt = (idx + 1 < total ? offsets[idx + 1] : (int32_t)n) - off;  // tile's digit-tid count
+  s_off[tid] = off;
+  s_loc[tid] = cnt;
+  s_run[tid] = 0;
+  __syncthreads();
+  // inclusive scan of the 256 digit counts (Hillis-Steele in LDS), then exclusive
+  for (int d = 1; d < kRadix; d <<= 1) {
+    const int v = tid >= d ? s_loc[tid - d] : 0;
+    __syncthreads();
+    s_loc[tid] += v;
+    __syncthreads();
+  }
+  s_loc[tid] -= cnt;
   const int64_t base = (int64_t)blockIdx.x * kSortTile;
+  // All of the tile's keys and values are loaded up front: the rounds below
+  // are separated by barriers, and a load issued per round would wait out one
+  // memory latency per round.
+  uint64_t kreg[kSortItems];
+  int32_t vreg[kSortItems];
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const int64_t i = base + (int64_t)r * kSortThreads + tid;
+    kreg[r] = i < n ? kin[i] : 0ull;
+    vreg[r] = i < n ? vin[i] : 0;
+  }
+  __syncthreads();
+#pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
     s_wcnt[0][tid] = 0;
     s_wcnt[1][tid] = 0;
@@ -183,8 +219,8 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     __syncthreads();
     const int64_t i = base + (int64_t)r * kSortThreads + tid;
     const bool valid = i < n;
-    const uint64_t key = valid ? kin[i] : 0ull;
-    const int32_t val = valid ? vin[i] : 0;
+    const uint64_t key = kreg[r];
+    const int32_t val = vreg[r];
     const int d = (int)((key >> shift) & (kRadix - 1));
     uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -199,13 +235,21 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     if (valid) {
       int pre = 0;
       for (int w = 0; w < wave; ++w) pre += s_wcnt[w][d];
-      const int64_t pos = (int64_t)s_off[d] + pre + rank;
-      kout[pos] = key;
-      vout[pos] = val;
+      const int lp = s_loc[d] + s_run[d] + pre + rank;
+      s_keys[lp] = key;
+      s_vals[lp] = val;
     }
     __syncthreads();
-    s_off[tid] += s_wcnt[0][tid] + s_wcnt[1][tid] + s_wcnt[2][tid] + s_wcnt[3][tid];
+    s_run[tid] += s_wcnt[0][tid] + s_wcnt[1][tid] + s_wcnt[2][tid] + s_wcnt[3][tid];
     __syncthreads();
+  }
+  const int tile_n = (int)(n - base < kSortTile ? n - base : kSortTile);
+  for (int j = tid; j < tile_n; j += kSortThreads) {
+    const uint64_t key = s_keys[j];
+    const int d = (int)((key >> shift) & (kRadix - 1));
+    const int64_t pos = (int64_t)s_off[d] + (j - s_loc[d]);
+    kout[pos] = key;
+    vout[pos] = s_vals[j];
   }
 }
 
