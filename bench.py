@@ -272,8 +272,8 @@ def main():
         copy_probe()
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this DTC-SpMM build runs on one GPU only; multi-GPU runs are not part of it "
-              "(DESIGN.md describes how the path would shard).", file=sys.stderr)
+        print(f"bench.py: --gpus {args.gpus}: this DTC-SpMM build runs on one GPU only; "
+              "multi-GPU runs are not part of it.", file=sys.stderr)
         sys.exit(2)
     if args.workload == "gcn":
         bench_gcn(args)
