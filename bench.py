@@ -9,7 +9,7 @@ A step is one SpMM launch C = A * B over the resident ME-TCF (built once on
 the GPU and timed separately, as the reference times preprocess and SpMM
 apart: DTCSpMM.cpp:337-348 vs DTCSpMM_kernel.cu:841-856).  The variant is the
 learned selector's pick (dtc/selector.py, weights in dtc/data/); every
-variant is also timed (exhaustive timing: the selector's judge and fallback)
+variant is also timed (exhaustive timing: the selector's yardstick and fallback)
 and its componentwise error against the float64 CSR product is recorded.
 The plain-FP32 path is the fastest FP32 variant of that table.  A matrix
 outside the selector's training range goes to exhaustive timing instead.
@@ -17,7 +17,7 @@ For the R-MAT workload the chosen variant is also timed on the TCA-reordered
 graph (field "reordered"; --no-reorder skips it); the headline value stays
 on the graph as generated.
 
-Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
+Prints one JSON line.  --gpus N > 1 exits: the library runs on one GPU.
 """
 from __future__ import annotations
 
@@ -272,7 +272,7 @@ def main():
         copy_probe()
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print(f"bench.py: --gpus {args.gpus}: this DTC-SpMM build runs on one GPU only; "
+        print(f"bench.py: --gpus {args.gpus}: this DTC-SpMM library runs on one GPU only; "
               "multi-GPU runs are not part of it.", file=sys.stderr)
         sys.exit(2)
     if args.workload == "gcn":
@@ -359,7 +359,7 @@ def main():
         log(f"selector pick: {sel.variant_name(sel.VARIANTS[pick])}"
             + (f" (outside its training range: {', '.join(ood)})" if ood else ""))
 
-    # ---- exhaustive table (the selector's judge and fallback)
+    # ---- exhaustive table (the selector's yardstick and fallback)
     table_variants = list(sel.VARIANTS)
     if not args.no_table:
         table_variants += [("balanced", p, 256, 4, 1, False) for p in EXTRA_PRECISIONS]

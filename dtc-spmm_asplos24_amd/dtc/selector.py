@@ -9,7 +9,7 @@ data are timings this library measures itself over seeded matrices of five
 families (``collect``); training and inference use the HIP kernels of
 csrc/selector.hip (``mlp_linear_forward``/``_backward``, ``mlp_mse``,
 ``adam_step``).  Exhaustive timing (``exhaustive``) is the fallback and the
-judge: regret = t(chosen) / t(best) - 1.
+yardstick: regret = t(chosen) / t(best) - 1.
 
     python -m dtc.selector collect --count 2000 --out timings.npz   # on a GPU
     python -m dtc.selector train --data timings.npz --out mlp.npz
