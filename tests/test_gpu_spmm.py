@@ -165,6 +165,28 @@ def test_balanced_empty_windows(ext, gappy, chunk, persistent):
     assert componentwise(C, C64, Cabs) <= TOL["fp32"], (chunk, persistent)
 
 
+@pytest.mark.parametrize("M", [1, 17, 100])
+def test_empty_matrix_writes_zeros(ext, M):
+    # No entries, so no TC blocks: every output row is an empty window and
+    # must be written as zeros over a NaN-filled output, by every schedule and
+    # precision, and by the reference's operator names.
+    rowptr = np.zeros(M + 1, np.int32)
+    case = Case(ext, rowptr, np.zeros(0, np.int32), M, np.zeros(0, np.float32), f"empty M={M}")
+    assert case.m["block_count"] == 0
+    B = torch.randn((M, 64), device="cuda")
+    for schedule in ("windows", "balanced"):
+        for precision in ("fp32", "bf16x6"):
+            say("spmm empty", M, schedule, precision)
+            out = torch.full((M, 64), float("nan"), device="cuda")
+            case.run(ext, B, schedule, precision, out=out)
+            assert (out == 0).all(), (M, schedule, precision)
+    d = case.dev
+    (C,) = ext.run_DTCSpMM(B, d[0], d[2], d[3], d[4], M, 0, "float4_split")
+    assert C.shape == (M, 64) and (C == 0).all()
+    (C,) = ext.run_DTCSpMM_balance(B, d[1], d[2], d[3], d[4], M, "float4_split")
+    assert C.shape == (M, 64) and (C == 0).all()
+
+
 def test_schedules_agree_rmat(ext, rmat12):
     rng = np.random.default_rng(5)
     Bn = rng.standard_normal((rmat12.M, 128)).astype(np.float32)
