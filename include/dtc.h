@@ -10,7 +10,9 @@
  * last failure.
  *
  * Index types follow the reference: int32 CSR, uint8 tile ids, int32 block
- * offsets and SparseAToX (DTC-SpMM/DTCSpMM.cpp:313-359).
+ * offsets and SparseAToX (DTC-SpMM/DTCSpMM.cpp:313-359).  Row counts, N,
+ * nnz and block counts are below 2^31; the dense operands B and C are
+ * addressed with 64-bit element offsets, so M x N may exceed 2^32 elements.
  */
 #ifndef DTC_H_
 #define DTC_H_
