@@ -160,18 +160,42 @@ __global__ __launch_bounds__(256) void k_sddmm(const int32_t *rowptr, const int3
     float p[kSddmmEdges];
 #pragma unroll
     for (int k = 0; k < kSddmmEdges; ++k) p[k] = 0.f;
-    for (int c = lane * VEC; c < n; c += kWave * VEC) {
-      vfloat xv[kSddmmEdges];
-      yvec yv[kSddmmEdges];
+    // bfloat16 y, all 8 edges in one row (the common case inside long rows):
+    // x is loaded once, so a lane moves 16 + 8*8 bytes instead of 8*(16 + 8).
+    // rk[] is wave-uniform, so this is a scalar branch; the general path is
+    // left as it was rather than copying registers whose loads are still in
+    // flight, which would make the loads wait on one another.  Not for
+    // float32 y: there the second path costs registers (one wave fewer per
+    // SIMD) and measured slower.
+    bool one_row = YBF;
 #pragma unroll
-      for (int k = 0; k < kSddmmEdges; ++k) {
-        xv[k] = *reinterpret_cast<const vfloat *>(x + (int64_t)rk[k] * n + c);
-        yv[k] = *reinterpret_cast<const yvec *>(y + (int64_t)jk[k] * n + c);
+    for (int k = 1; k < kSddmmEdges; ++k) one_row = one_row && rk[k] == rk[0];
+    if (one_row) {
+      for (int c = lane * VEC; c < n; c += kWave * VEC) {
+        const vfloat xv = *reinterpret_cast<const vfloat *>(x + (int64_t)rk[0] * n + c);
+        yvec yv[kSddmmEdges];
+#pragma unroll
+        for (int k = 0; k < kSddmmEdges; ++k)
+          yv[k] = *reinterpret_cast<const yvec *>(y + (int64_t)jk[k] * n + c);
+#pragma unroll
+        for (int k = 0; k < kSddmmEdges; ++k)
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) p[k] += xv[v] * YV::at(yv[k], v);
       }
+    } else {
+      for (int c = lane * VEC; c < n; c += kWave * VEC) {
+        vfloat xv[kSddmmEdges];
+        yvec yv[kSddmmEdges];
 #pragma unroll
-      for (int k = 0; k < kSddmmEdges; ++k)
+        for (int k = 0; k < kSddmmEdges; ++k) {
+          xv[k] = *reinterpret_cast<const vfloat *>(x + (int64_t)rk[k] * n + c);
+          yv[k] = *reinterpret_cast<const yvec *>(y + (int64_t)jk[k] * n + c);
+        }
 #pragma unroll
-        for (int v = 0; v < VEC; ++v) p[k] += xv[k][v] * YV::at(yv[k], v);
+        for (int k = 0; k < kSddmmEdges; ++k)
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) p[k] += xv[k][v] * YV::at(yv[k], v);
+      }
     }
 #pragma unroll
     for (int k = 0; k < kSddmmEdges; ++k)
