@@ -4,7 +4,7 @@ import pytest
 import torch
 
 import oracle
-from helpers import assert_metcf_equal, gpu_metcf, load_fixture_18, naive_metcf, random_csr, say
+from helpers import assert_metcf_equal, gpu_metcf, load_fixture_18, random_csr, say
 
 pytestmark = pytest.mark.gpu
 
