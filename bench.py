@@ -34,6 +34,9 @@ sys.path[:0] = [PKG]
 ERR_TARGET = 1e-5
 HBM_PEAK_GBS = 8000.0
 EXTRA_PRECISIONS = ["bf16x3", "fp16", "bf16"]  # timed and reported, fail the error gate
+# Chunks below the selector's range (32-column tiles): many more, shorter
+# waves, which pays on small matrices such as the 4096-row uniform case.
+EXTRA_SMALL_CHUNKS = [16, 8]
 
 
 def log(*a):
@@ -252,7 +255,8 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--scale", type=int, default=20)
-    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--n", type=int, default=0,
+                    help="columns of B (default: 256 for rmat and gcn, 128 for uniform and sbm)")
     ap.add_argument("--variant", default="selector",
                     help="selector (learned pick), exhaustive, or schedule:precision:chunk:vec:subtiles")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -266,11 +270,14 @@ def main():
     ap.add_argument("--no-reorder", action="store_true",
                     help="skip timing the chosen variant on the TCA-reordered graph (R-MAT only)")
     ap.add_argument("--chunks", default="",
-                    help="extra balanced FP32 chunk sizes for the table, e.g. 16,4096")
+                    help="extra balanced FP32 variants for the table: chunk sizes, optionally "
+                         "with the lane width as chunk:vec, e.g. 16,4096 or 8:2,16:2 (vec 4 if omitted)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
         return
+    if args.n <= 0:
+        args.n = 128 if args.workload in ("uniform", "sbm") else 256
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(f"bench.py: --gpus {args.gpus}: this DTC-SpMM library runs on one GPU only; "
               "multi-GPU runs are not part of it.", file=sys.stderr)
@@ -322,9 +329,16 @@ def main():
     sum_u = A.sum_unique_columns()
     log(f"ME-TCF: T={T} W={W} sumU={sum_u} build {np.median(pre_ms):.2f} ms")
 
-    gen = torch.Generator(device=dev)
-    gen.manual_seed(2)
-    B = torch.randn((M, N), generator=gen, device=dev, dtype=torch.float32)
+    if args.workload == "uniform":
+        # the small plain case: B U[-1, 1] from PCG64(1)
+        rng_b = np.random.Generator(np.random.PCG64(1))
+        B = torch.from_numpy(rng_b.uniform(-1.0, 1.0, (M, N)).astype(np.float32)).to(dev)
+        bdesc = "U[-1,1] (PCG64 seed 1)"
+    else:
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(2)
+        B = torch.randn((M, N), generator=gen, device=dev, dtype=torch.float32)
+        bdesc = "N(0,1) (torch seed 2)"
     C = torch.empty((M, N), device=dev, dtype=torch.float32)
 
     # ---- float64 reference for the error gate
@@ -363,7 +377,10 @@ def main():
     table_variants = list(sel.VARIANTS)
     if not args.no_table:
         table_variants += [("balanced", p, 256, 4, 1, False) for p in EXTRA_PRECISIONS]
-        table_variants += [("balanced", "fp32", int(c), 4, 1, False) for c in args.chunks.split(",") if c]
+        table_variants += [("balanced", "fp32", c, 2, 1, False) for c in EXTRA_SMALL_CHUNKS]
+        for c in filter(None, args.chunks.split(",")):
+            chunk, _, vec = c.partition(":")
+            table_variants.append(("balanced", "fp32", int(chunk), int(vec or 4), 1, False))
     table = []
     err_cache = {}
     if not args.no_table:
@@ -535,7 +552,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": dtypes[chosen_v[1]],
-        "data": f"synthetic (seeded {args.workload} matrix, B from torch N(0,1) seed 2)",
+        "data": f"synthetic (seeded {args.workload} matrix, B {bdesc})",
         "config": {"workload": f"{wdesc}; B {M}x{N} f32",
                    "num_rows": M, "nnz": nnz, "N": N, "tc_blocks": T, "row_windows": W,
                    "sum_unique_columns": sum_u, "variant": sel.variant_name(chosen_v), "chosen_by": how,
