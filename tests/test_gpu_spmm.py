@@ -187,6 +187,30 @@ def test_empty_matrix_writes_zeros(ext, M):
     assert C.shape == (M, 64) and (C == 0).all()
 
 
+def test_dtc_bench_timed_loop(ext, ragged, tmp_path, monkeypatch):
+    """DTC_BENCH=1 reproduces the reference's 1000 timed launches and its CSV
+    record "DTCSpMM:,<ms>,<GFLOP/s>," (DTCSpMM_kernel.cu:931-937), appended
+    without a newline, for both operator names; the result is that of one
+    plain launch."""
+    rng = np.random.default_rng(77)
+    B = torch.from_numpy(rng.standard_normal((ragged.M, 64)).astype(np.float32)).cuda()
+    d = ragged.dev
+    nnz = int(ragged.col.shape[0])
+    (Cw1,) = ext.run_DTCSpMM(B, d[0], d[2], d[3], d[4], ragged.M, nnz, "float4_split")
+    (Cb1,) = ext.run_DTCSpMM_balance(B, d[1], d[2], d[3], d[4], ragged.M, "float4_split")
+    monkeypatch.chdir(tmp_path)
+    monkeypatch.setenv("DTC_BENCH", "1")
+    say("DTC_BENCH loop")
+    (Cw2,) = ext.run_DTCSpMM(B, d[0], d[2], d[3], d[4], ragged.M, nnz, "float4_split")
+    (Cb2,) = ext.run_DTCSpMM_balance(B, d[1], d[2], d[3], d[4], ragged.M, "float4_split")
+    fields = (tmp_path / "DTCSpMM_exe_time_and_throughput.csv").read_text().split(",")
+    assert len(fields) == 7 and fields[0] == fields[3] == "DTCSpMM:" and fields[6] == ""
+    for ms, gflops in ((float(fields[1]), float(fields[2])), (float(fields[4]), float(fields[5]))):
+        assert ms > 0 and abs(gflops - 2.0 * nnz * 64 / (ms * 1e6)) <= 1e-3 * gflops
+    assert torch.equal(Cw1, Cw2)
+    assert torch.allclose(Cb1, Cb2, rtol=1e-5, atol=1e-5)  # cut windows add in run order
+
+
 def test_schedules_agree_rmat(ext, rmat12):
     rng = np.random.default_rng(5)
     Bn = rng.standard_normal((rmat12.M, 128)).astype(np.float32)
