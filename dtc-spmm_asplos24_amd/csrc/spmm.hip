@@ -288,6 +288,7 @@ struct GroupMeta {
   int tile;                // tile id of element f0 + lane (raw)
   float val;               // its value (raw)
   int rid;                 // window of block bb + lane, -1 at or past the run's end
+  int fe;                  // without WALK: TCblock_offset[min(bb + 4, b1)], the group's end
 };
 
 // Loads are unconditional and their results are only consumed one group
@@ -320,6 +321,8 @@ __device__ __forceinline__ void load_meta(const SpmmParams &p, int bb, int b1, b
   if constexpr (WALK) {
     const int rb = bb + lane;
     m.rid = rb < b1 ? p.rowid[rb] : -1;
+  } else {
+    m.fe = p.off[min(bb + kGroup, b1)];
   }
 }
 
@@ -328,14 +331,23 @@ __device__ __forceinline__ void load_meta(const SpmmParams &p, int bb, int b1, b
 // at the run's end, where the look-ahead holds -1); else at b1.
 template <bool WALK>
 __device__ __forceinline__ void resolve_meta(const SpmmParams &p, int b1, GroupMeta &m) {
-  int nblk;
-  if constexpr (WALK) {
-    const int prev = __shfl_up(m.rid, 1, 64);
-    const uint64_t starts = __ballot(lane_id() > 0 && m.rid != prev);
-    nblk = starts ? min(kGroup, (int)__builtin_ctzll(starts)) : kGroup;
-  } else {
-    nblk = min(kGroup, b1 - m.bb);
+  if constexpr (!WALK) {
+    // The group's end offset was loaded with the rest (its length is known
+    // up front); offsets past it are clamped to it, which for non-decreasing
+    // offsets is the end itself -- no index into the offsets by a run-time
+    // count, which the compiler would place in scratch memory.
+    m.nblk = min(kGroup, b1 - m.bb);
+    const int o4 = min(m.fe, p.nnz);
+    m.o0 = min(m.f0, o4);
+    m.o1 = min(m.f1, o4);
+    m.o2 = min(m.f2, o4);
+    m.o3 = min(m.f3, o4);
+    m.o4 = o4;
+    return;
   }
+  const int prev = __shfl_up(m.rid, 1, 64);
+  const uint64_t starts = __ballot(lane_id() > 0 && m.rid != prev);
+  const int nblk = starts ? min(kGroup, (int)__builtin_ctzll(starts)) : kGroup;
   // selects, not an indexed array (which would live in scratch memory)
   auto f = [&m](int k) {
     return k == 0 ? m.f0 : k == 1 ? m.f1 : k == 2 ? m.f2 : k == 3 ? m.f3 : m.f4;
@@ -655,11 +667,25 @@ __device__ __forceinline__ void run_unit(const SpmmParams &p, int item, int ct, 
   }
 }
 
+// Waves per SIMD the compiler must leave room for (its register budget),
+// per instance.  Left to itself it keeps the balanced kernels' accumulators
+// in AGPRs and stops at 5 waves with 32-column tiles; asked for 7 it fits
+// them in 70 VGPRs without scratch, and the SpMM runs ~5% faster (R-MAT
+// scale 20: 4.23 -> 4.00 ms at N = 256, 0.60 -> 0.56 ms at N = 32).  With
+// 64-column tiles, 5 waves instead of 4 measured no faster; the bf16x3 and
+// bf16x6 splits would spill.  Those keep the default.
+template <int PREC, int VEC, int NSUB, bool BALANCED>
+constexpr int spmm_waves_per_eu() {
+  if (!BALANCED || PREC == DTC_PRECISION_BF16X3 || PREC == DTC_PRECISION_BF16X6) return 1;
+  return VEC * NSUB <= 2 ? 7 : 1;
+}
+
 // Workgroup i serves column tile i % n_ct.  Workgroups are dispatched to the
 // 8 XCDs round robin, so when n_ct divides 8 every XCD (and its L2) only
 // sees one column tile of B: 1/n_ct of each gathered row.
 template <int PREC, int VEC, int NSUB, bool BALANCED>
-__global__ __launch_bounds__(256) void spmm_kernel(SPMM_KERNEL_ARGS) {
+__global__ __launch_bounds__(256) __attribute__((
+    amdgpu_waves_per_eu(spmm_waves_per_eu<PREC, VEC, NSUB, BALANCED>()))) void spmm_kernel(SPMM_KERNEL_ARGS) {
   const SpmmParams p = SPMM_PARAMS_FROM_ARGS;
   __shared__ __attribute__((aligned(16))) float s_tile[kWavesPerBlock][wave_lds_floats<VEC, NSUB>()];
   const int wave = wave_uniform(threadIdx.x >> 6);
@@ -672,7 +698,8 @@ __global__ __launch_bounds__(256) void spmm_kernel(SPMM_KERNEL_ARGS) {
 // Persistent variant of the balanced schedule: every wave pulls units from a
 // global counter until the list is exhausted.
 template <int PREC, int VEC, int NSUB>
-__global__ __launch_bounds__(256) void spmm_kernel_persistent(SPMM_KERNEL_ARGS) {
+__global__ __launch_bounds__(256) __attribute__((
+    amdgpu_waves_per_eu(spmm_waves_per_eu<PREC, VEC, NSUB, true>()))) void spmm_kernel_persistent(SPMM_KERNEL_ARGS) {
   const SpmmParams p = SPMM_PARAMS_FROM_ARGS;
   __shared__ __attribute__((aligned(16))) float s_tile[kWavesPerBlock][wave_lds_floats<VEC, NSUB>()];
   const int wave = wave_uniform(threadIdx.x >> 6);
