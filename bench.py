@@ -442,7 +442,7 @@ def main():
         fr = runner(v32)
         for _ in range(3):
             fr()
-        fp32_ms = timed(stream, fr, args.steps)
+        fp32_ms, _ = timed_steps(stream, fr, args.steps)  # per-step events, as the chosen one
         fp32 = {"variant": sel.variant_name(v32), "ms": round(fp32_ms, 4),
                 "gflops": round(flops / fp32_ms / 1e6, 2), "speedup_of_chosen": round(fp32_ms / ms, 3)}
 
