@@ -49,6 +49,7 @@ class DTCMatrix:
         self.values_tc = None if self.values is None else ext.gather(self.values, self.tc_edge)
         self._transpose = None
         self._perm_t = None
+        self._max_window_blocks = None
 
     # ------------------------------------------------------------------ info
     def sum_unique_columns(self) -> int:
@@ -66,11 +67,25 @@ class DTCMatrix:
         return int((full + real_last).sum().item())
 
     # ----------------------------------------------------------------- SpMM
-    def spmm(self, B: torch.Tensor, schedule: str = "balanced", precision: str = "fp32",
+    @property
+    def max_window_blocks(self) -> int:
+        """TC blocks of the longest row window (computed once)."""
+        if self._max_window_blocks is None:
+            rwo = self.row_window_offset
+            self._max_window_blocks = int((rwo[1:] - rwo[:-1]).max().item()) if rwo.numel() > 1 else 0
+        return self._max_window_blocks
+
+    def spmm(self, B: torch.Tensor, schedule: str = "auto", precision: str = "fp32",
              values_tc: torch.Tensor | None = None, use_values: bool = True, vec: int = 0,
              subtiles: int = 0, chunk_blocks: int = 0, persistent: bool = False,
              out: torch.Tensor | None = None) -> torch.Tensor:
+        """C = A * B.  schedule "auto" runs the unbalanced schedule when no row
+        window is longer than ``dtc.selector.AUTO_WINDOW_BLOCKS`` TC blocks
+        and the balanced one otherwise; "windows" / "balanced" force one."""
         ext = load_extension()
+        if schedule == "auto":
+            from .selector import default_schedule
+            schedule = default_schedule(self.max_window_blocks)
         vals = values_tc if values_tc is not None else (self.values_tc if use_values else None)
         idx = self.row_window_offset if schedule in ("windows", "unbalanced") else self.tcblock_rowid
         return ext.spmm(B.contiguous(), idx, self.tcblocktile_id, self.tcblock_offset,
@@ -125,7 +140,7 @@ class _SpMMFunction(torch.autograd.Function):
         return d_values, d_B, None, None, None
 
 
-def spmm_autograd(values: torch.Tensor, B: torch.Tensor, A: DTCMatrix, schedule: str = "balanced",
+def spmm_autograd(values: torch.Tensor, B: torch.Tensor, A: DTCMatrix, schedule: str = "auto",
                   precision: str = "fp32") -> torch.Tensor:
     """C = A(values) * B, differentiable in values (CSR order) and B."""
     return _SpMMFunction.apply(values, B, A, schedule, precision)

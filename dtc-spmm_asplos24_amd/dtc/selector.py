@@ -66,11 +66,32 @@ def default_chunk_blocks(num_blocks: int) -> int:
     return c
 
 
-def default_variant(num_blocks: int) -> int:
+# Longest row window (TC blocks) up to which the default schedule is the
+# unbalanced one: one wave per window beats the balanced chunks when no
+# window is long (uniform, banded, block-diagonal matrices); past it, hub
+# windows would serialise.  Chosen on the training split of the selector's
+# timings (128-192 blocks are equivalent there).
+AUTO_WINDOW_BLOCKS = 160
+
+
+def default_schedule(max_window_blocks: int) -> str:
+    """The schedule of ``DTCMatrix.spmm(schedule="auto")``."""
+    return "windows" if int(max_window_blocks) <= AUTO_WINDOW_BLOCKS else "balanced"
+
+
+def default_variant(num_blocks: int, max_window_blocks: int) -> int:
     """Index in VARIANTS of what the library runs with default options (the
-    method without the selector): balanced, FP32, 32-column tiles, the
-    default chunk."""
+    method without the selector): FP32, unbalanced with 64-column tiles when
+    no window is longer than AUTO_WINDOW_BLOCKS, else balanced with 32-column
+    tiles and the default chunk."""
+    if default_schedule(max_window_blocks) == "windows":
+        return VARIANTS.index(("windows", "fp32", 0, 4, 1, False))
     return VARIANTS.index(("balanced", "fp32", default_chunk_blocks(int(num_blocks)), 2, 1, False))
+
+
+def max_window_blocks_of(features_row) -> int:
+    """The longest window's block count from a feature vector."""
+    return int(round(2.0 ** float(features_row[FEATURE_NAMES.index("log2_max_blocks_per_window")]) - 1.0))
 
 FEATURE_NAMES = [
     "log2_rows", "log2_nnz", "log2_N", "log2_avg_deg", "deg_cv", "log2_max_deg",
@@ -400,7 +421,8 @@ def evaluate(model: MLPSelector, data, holdout_from: int = 1600) -> dict:
     X, t = data["features"][test], data["times_ms"][test]
     chosen = model.choose(X)
     fixed = best_fixed_variant(data["times_ms"][train])
-    dflt = np.array([default_variant(b) for b in data["blocks"][test]])
+    dflt = np.array([default_variant(b, max_window_blocks_of(x))
+                     for b, x in zip(data["blocks"][test], X)])
     r_sel = regret(t, chosen)
     r_def = regret(t, dflt)
     r_fix = regret(t, np.full(t.shape[0], fixed))
@@ -411,7 +433,8 @@ def evaluate(model: MLPSelector, data, holdout_from: int = 1600) -> dict:
             "fallback_to_exhaustive": int(fallback.sum()),
             "guarded_mean_regret": float(r_guard.mean()),
             "selector_mean_regret": float(r_sel.mean()), "selector_p90_regret": float(np.quantile(r_sel, 0.9)),
-            "default_variant": "balanced:fp32:c<T/8192 in 32..512>:v2x1",
+            "default_variant": (f"windows:fp32:c0:v4x1 if the longest window has <= {AUTO_WINDOW_BLOCKS} "
+                                "blocks, else balanced:fp32:c<T/8192 in 32..512>:v2x1"),
             "default_mean_regret": float(r_def.mean()), "default_p90_regret": float(np.quantile(r_def, 0.9)),
             "best_fixed_variant": variant_name(VARIANTS[fixed]),
             "best_fixed_mean_regret": float(r_fix.mean()), "best_fixed_p90_regret": float(np.quantile(r_fix, 0.9)),

@@ -211,6 +211,34 @@ def test_dtc_bench_timed_loop(ext, ragged, tmp_path, monkeypatch):
     assert torch.allclose(Cb1, Cb2, rtol=1e-5, atol=1e-5)  # cut windows add in run order
 
 
+def test_auto_schedule(ext):
+    """DTCMatrix.spmm's default schedule: unbalanced when no row window holds
+    more than AUTO_WINDOW_BLOCKS TC blocks (here 2 per window), balanced when
+    one does (a 3,000-column row: 375 blocks); same result as the forced
+    schedule, within float32 against float64."""
+    from dtc import selector as sel
+    from dtc.matrix import DTCMatrix
+    rng = np.random.default_rng(12)
+    M = 4096
+    rowptr, col = random_csr(rng, M, M, 6.0)
+    short = (rowptr, col)
+    rows = [list(col[rowptr[r]:rowptr[r + 1]]) for r in range(M)]
+    rows[77] = sorted(set(rows[77]) | set(rng.choice(M, 3000, replace=False).tolist()))
+    rp = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int32)
+    long_ = (rp, np.concatenate([np.array(r, np.int32) for r in rows]).astype(np.int32))
+    for (rowptr_, col_), want in ((short, "windows"), (long_, "balanced")):
+        vals = rng.uniform(-1, 1, col_.shape[0]).astype(np.float32)
+        A = DTCMatrix(rowptr_, col_, vals, num_rows=M, num_cols=M)
+        assert sel.default_schedule(A.max_window_blocks) == want, A.max_window_blocks
+        Bn = rng.standard_normal((M, 64)).astype(np.float32)
+        B = torch.from_numpy(Bn).cuda()
+        say("auto schedule", want, A.max_window_blocks)
+        C = A.spmm(B)
+        assert torch.allclose(C, A.spmm(B, schedule=want), rtol=1e-5, atol=1e-5)
+        C64, Cabs = oracle.spmm_csr_f64(rowptr_, col_, vals, Bn, M, with_abs=True)
+        assert componentwise(C.cpu().numpy(), C64, Cabs) <= TOL["fp32"]
+
+
 def test_schedules_agree_rmat(ext, rmat12):
     rng = np.random.default_rng(5)
     Bn = rng.standard_normal((rmat12.M, 128)).astype(np.float32)

@@ -84,3 +84,28 @@ def test_model_range_guard():
     x[sel.FEATURE_NAMES.index("log2_blocks")] = np.log2(1.04e7)
     assert sel.out_of_range(x, z["lo"], z["hi"]) == ["log2_nnz", "log2_blocks"]
     assert sel.out_of_range(x, None, None) == []
+
+
+def test_default_schedule_rule():
+    """The default options' schedule: unbalanced up to AUTO_WINDOW_BLOCKS
+    blocks in the longest window, balanced past it; on the held-out timings
+    the rule beats balanced-only defaults."""
+    from dtc import selector as sel
+    from dtc.selector import FEATURE_NAMES
+    assert sel.default_schedule(0) == "windows"
+    assert sel.default_schedule(sel.AUTO_WINDOW_BLOCKS) == "windows"
+    assert sel.default_schedule(sel.AUTO_WINDOW_BLOCKS + 1) == "balanced"
+    assert sel.VARIANTS[sel.default_variant(10_000, 12)] == ("windows", "fp32", 0, 4, 1, False)
+    assert sel.VARIANTS[sel.default_variant(10_000_000, 5_000)] == ("balanced", "fp32", 512, 2, 1, False)
+    x = np.zeros(len(FEATURE_NAMES), np.float32)
+    x[FEATURE_NAMES.index("log2_max_blocks_per_window")] = np.log2(161.0)
+    assert sel.max_window_blocks_of(x) == 160
+    z = np.load(sel.DEFAULT_DATA)
+    test = z["seed"] >= 1600
+    t, X, blocks = z["times_ms"][test], z["features"][test], z["blocks"][test]
+    best = np.nanmin(t, axis=1)
+    rows = np.arange(t.shape[0])
+    rule = np.array([sel.default_variant(b, sel.max_window_blocks_of(x)) for b, x in zip(blocks, X)])
+    balanced_only = np.array([sel.VARIANTS.index(("balanced", "fp32", sel.default_chunk_blocks(int(b)), 2, 1, False))
+                              for b in blocks])
+    assert (t[rows, rule] / best).mean() < (t[rows, balanced_only] / best).mean()
