@@ -371,7 +371,7 @@ std::vector<Tensor> run_balanced(const Operands &o, const Tensor &rowid, int64_t
   // DTCSpMM_kernel.cu:2687); the general entry takes B's own row count.
   const int64_t b_rows = ref_shape ? std::min<int64_t>(num_nodes, o.input.size(0)) : o.input.size(0);
   size_t ws_bytes = 0;
-  check_status(dtc_spmm_balanced_workspace_size((int32_t)T, &ws_bytes), "run_DTCSpMM_balance");
+  check_status(dtc_spmm_balanced_workspace_size_ex((int32_t)T, (int32_t)n, &opt, &ws_bytes), "run_DTCSpMM_balance");
   Tensor ws = torch::empty({(int64_t)ws_bytes}, o.input.options().dtype(torch::kUInt8));
   const float *vals = o.values.has_value() ? o.values->data_ptr<float>() : nullptr;
   auto launch = [&]() {
@@ -405,13 +405,14 @@ std::vector<Tensor> run_DTCSpMM(Tensor input, Tensor Rowwindow_offset, Tensor TC
 std::vector<Tensor> run_DTCSpMM_balance(Tensor input, Tensor TCblock_rowid, Tensor TCblocktile_id,
                                         Tensor TCblock_offset, Tensor sparse_AToX_idx, int64_t num_nodes,
                                         std::string exeplan, c10::optional<Tensor> values, std::string precision,
-                                        int64_t chunk_blocks, bool persistent) {
+                                        int64_t chunk_blocks, bool persistent, bool deterministic) {
   CHECK_DTYPE(input, torch::kFloat32);
   dtc_spmm_options_t opt{};
   parse_exeplan(exeplan, opt);
   opt.precision = parse_precision(precision);
   opt.chunk_blocks = (int32_t)chunk_blocks;
   opt.persistent = persistent ? 1 : 0;
+  opt.deterministic = deterministic ? 1 : 0;
   Operands o{input, TCblocktile_id, TCblock_offset, sparse_AToX_idx, values};
   return run_balanced(o, TCblock_rowid, num_nodes, opt, true, true, c10::nullopt);
 }
@@ -437,8 +438,10 @@ std::vector<Tensor> DTCSpMM_gcn(Tensor input, Tensor Rowwindow_offset, Tensor TC
 // optional preallocated output, no CSV side effects.
 Tensor spmm(Tensor input, Tensor window_or_rowid, Tensor TCblocktile_id, Tensor TCblock_offset, Tensor sparse_AToX_idx,
             int64_t num_nodes, std::string schedule, c10::optional<Tensor> values, std::string precision, int64_t vec,
-            int64_t subtiles, int64_t chunk_blocks, bool persistent, c10::optional<Tensor> out) {
+            int64_t subtiles, int64_t chunk_blocks, bool persistent, c10::optional<Tensor> out,
+            bool deterministic) {
   dtc_spmm_options_t opt{};
+  opt.deterministic = deterministic ? 1 : 0;
   opt.precision = parse_precision(precision);
   TORCH_CHECK_VALUE(vec == 0 || vec == 1 || vec == 2 || vec == 4 || vec == 8, "vec must be 0, 1, 2, 4 or 8");
   TORCH_CHECK_VALUE(subtiles >= 0 && subtiles <= 2, "subtiles must be 0, 1 or 2");
@@ -774,13 +777,14 @@ PYBIND11_MODULE(DTCSpMM, m) {
   m.def("run_DTCSpMM_balance", &run_DTCSpMM_balance, "DTC-SpMM forward, balanced TC-block chunks", py::arg("input"),
         py::arg("TCblock_rowid"), py::arg("TCblocktile_id"), py::arg("TCblock_offset"), py::arg("sparse_AToX_idx"),
         py::arg("num_nodes"), py::arg("exeplan"), py::arg("values") = py::none(), py::arg("precision") = "fp32",
-        py::arg("chunk_blocks") = 0, py::arg("persistent") = false);
+        py::arg("chunk_blocks") = 0, py::arg("persistent") = false, py::arg("deterministic") = false);
   m.def("DTCSpMM_gcn", &DTCSpMM_gcn, "DTC-SpMM forward, single launch, variant chosen from N", py::arg("input"),
         py::arg("Rowwindow_offset"), py::arg("TCblocktile_id"), py::arg("TCblock_offset"), py::arg("sparse_AToX_idx"),
         py::arg("num_nodes"), py::arg("num_edges"), py::arg("values") = py::none());
   // The reference binds these names to the forward functions
-  // (DTCSpMM.cpp:386-393); kept as the same aliases.  True gradients are in
-  // the dtc.autograd module.
+  // (DTCSpMM.cpp:386-393); kept as the same aliases.  True gradients are
+  // dtc.matrix.spmm_autograd (dB through the transposed ME-TCF, dvalues by
+  // SDDMM).
   m.def("backward_ptx_uint8_improved", &run_DTCSpMM, "alias of run_DTCSpMM (reference behaviour)", py::arg("input"),
         py::arg("Rowwindow_offset"), py::arg("TCblocktile_id"), py::arg("TCblock_offset"), py::arg("sparse_AToX_idx"),
         py::arg("num_nodes"), py::arg("num_edges"), py::arg("exeplan"), py::arg("values") = py::none(),
@@ -791,12 +795,13 @@ PYBIND11_MODULE(DTCSpMM, m) {
   m.def("balance_backward_ptx_uint8_prefetch", &run_DTCSpMM_balance, "alias of run_DTCSpMM_balance",
         py::arg("input"), py::arg("TCblock_rowid"), py::arg("TCblocktile_id"), py::arg("TCblock_offset"),
         py::arg("sparse_AToX_idx"), py::arg("num_nodes"), py::arg("exeplan"), py::arg("values") = py::none(),
-        py::arg("precision") = "fp32", py::arg("chunk_blocks") = 0, py::arg("persistent") = false);
+        py::arg("precision") = "fp32", py::arg("chunk_blocks") = 0, py::arg("persistent") = false,
+        py::arg("deterministic") = false);
   m.def("spmm", &spmm, "SpMM with explicit schedule and tuning knobs", py::arg("input"), py::arg("window_or_rowid"),
         py::arg("TCblocktile_id"), py::arg("TCblock_offset"), py::arg("sparse_AToX_idx"), py::arg("num_nodes"),
         py::arg("schedule") = "balanced", py::arg("values") = py::none(), py::arg("precision") = "fp32",
         py::arg("vec") = 0, py::arg("subtiles") = 0, py::arg("chunk_blocks") = 0, py::arg("persistent") = false,
-        py::arg("out") = py::none());
+        py::arg("out") = py::none(), py::arg("deterministic") = false);
   m.def("csr_transpose", &csr_transpose, "CSR transpose on the GPU -> (rowptr_t, col_t, perm)",
         py::arg("nodePointer"), py::arg("edgeList"), py::arg("num_rows"), py::arg("num_cols"));
   m.def("sddmm_csr", &sddmm_csr, "out[e] = X[row(e)] . Y[col(e)]", py::arg("nodePointer"), py::arg("edgeList"),

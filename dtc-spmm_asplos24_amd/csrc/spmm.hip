@@ -48,6 +48,13 @@ template <int VEC, int NSUB>
 constexpr int wave_lds_floats() {
   return kBlkH * 16 * VEC * NSUB > kTileFloats ? kBlkH * 16 * VEC * NSUB : kTileFloats;
 }
+// ... followed by 4 words of wave state (the balanced chunk index for the
+// deterministic merge), kept in LDS rather than in registers: the flush
+// that reads it runs at the walk loop's register peak.
+template <int VEC, int NSUB>
+constexpr int wave_lds_words() {
+  return wave_lds_floats<VEC, NSUB>() + 4;
+}
 constexpr int kWavesPerBlock = 4;
 
 struct SpmmParams {
@@ -70,6 +77,7 @@ struct SpmmParams {
   int32_t chunk;       // balanced chunk length in TC blocks
   int32_t *queue;      // persistent work counter
   const int2 *runs;    // balanced: run [b, e) of every chunk, cut flags in the sign bits
+  float *parts;        // balanced, deterministic merge: partial-tile slots (NULL: atomics)
 };
 
 // Kernels take the arrays as restrict-qualified arguments: the compiler may
@@ -84,7 +92,7 @@ struct SpmmParams {
   SpmmParams {                                                                             \
     a_rwo, a_rowid, a_tile, a_off, a_atox, a_vals, a_B, a_C, a_p.num_windows,              \
         a_p.num_blocks, a_p.M, a_p.b_rows, a_p.N, a_p.nnz, a_p.n_ct, a_p.num_items,        \
-        a_p.chunk, a_p.queue, a_p.runs                                                     \
+        a_p.chunk, a_p.queue, a_p.runs, a_p.parts                                          \
   }
 
 template <int VEC>
@@ -416,15 +424,18 @@ __device__ __forceinline__ void build_a(const SpmmParams &p, const GroupMeta &m,
   t4[lane * 2] = z4;
   t4[lane * 2 + 1] = z4;
   wave_lds_sync();
+  // Stored elements are added into the tile (LDS float atomics): a CSR with
+  // a repeated (row, column) pair puts several elements on one tile id, and
+  // the product must count each of them, as the CSR product does.
   if (m.o0 + lane < m.o4) {
     const int e = m.o0 + lane;
     const int gg = (e >= m.o1) + (e >= m.o2) + (e >= m.o3);
-    lds[gg * (kBlkH * kBlkW) + (m.tile & (kBlkH * kBlkW - 1))] = m.val;
+    atomicAdd(&lds[gg * (kBlkH * kBlkW) + (m.tile & (kBlkH * kBlkW - 1))], m.val);
   }
   for (int e = m.o0 + 64 + lane; e < m.o4; e += 64) {
     const int gg = (e >= m.o1) + (e >= m.o2) + (e >= m.o3);
     const float v = has_vals ? p.vals[e] : 1.0f;
-    lds[gg * (kBlkH * kBlkW) + (p.tile[e] & (kBlkH * kBlkW - 1))] = v;
+    atomicAdd(&lds[gg * (kBlkH * kBlkW) + (p.tile[e] & (kBlkH * kBlkW - 1))], v);
   }
   wave_lds_sync();
   const floatx4 lo = t4[(g * (kBlkH * kBlkW) + q * kBlkW) / 4];
@@ -529,9 +540,15 @@ __device__ __forceinline__ void zero_windows(const SpmmParams &p, int w0, int w1
 // row-major, so that each atomic instruction covers 64 consecutive floats of
 // a row (or two rows of a 32-column tile) instead of lanes 16 bytes apart
 // spread over 4 rows -- a quarter of the cache lines per instruction.
+// With the deterministic merge (p.parts set) the row-major tile is stored
+// instead, whole, in a partial-tile slot: slot 2 c holds chunk c's share of
+// its first window, slot 2 c + 1 of its last (`last_role`), each slot n_ct
+// tiles of 16 x TW floats; k_balance_fixup sums them in chunk order.  The
+// chunk index is the one run_unit left in the wave's LDS state word.
 template <int VEC, int NSUB>
 __device__ __forceinline__ void add_window_atomic(const SpmmParams &p, int w, int colbase,
-                                                  float *lds, const floatx4 (&acc)[NSUB][VEC]) {
+                                                  float *lds, const floatx4 (&acc)[NSUB][VEC],
+                                                  bool last_role) {
   constexpr int TW = 16 * VEC * NSUB;  // tile width in floats
   const int lane = lane_id();
   const int g = lane >> 4;
@@ -545,6 +562,16 @@ __device__ __forceinline__ void add_window_atomic(const SpmmParams &p, int w, in
   wave_lds_sync();
   const int col0 = colbase - q * VEC;  // first column of the tile
   const int64_t N = p.N;
+  if (p.parts) {
+    const int item = wave_uniform(reinterpret_cast<const int *>(lds)[wave_lds_floats<VEC, NSUB>()]);
+    const int slot = 2 * item + (last_role ? 1 : 0);
+    const int ct = wave_uniform(col0) / TW;
+    float *dst = p.parts + ((int64_t)slot * p.n_ct + ct) * (kBlkH * TW);
+#pragma unroll 1
+    for (int k = 0; k < kBlkH * TW / 64; ++k) dst[k * 64 + lane] = lds[k * 64 + lane];
+    wave_lds_sync();
+    return;
+  }
   // Rolled: at most two flushes per chunk take this path, and unrolling it
   // keeps every address live at once (twice the kernel's registers).
 #pragma unroll 1
@@ -592,7 +619,8 @@ __device__ __forceinline__ void walk_blocks(const SpmmParams &p, int b, int e, i
     // would hold up the wait for the gathers.
     const int w = wave_uniform(cur.rid);  // window of block cur.bb (lane 0)
     if (w != cw) {
-      if (cw == atomic_w0 || cw == atomic_w1) add_window_atomic<VEC, NSUB>(p, cw, colbase, lds, acc);
+      if (cw == atomic_w0 || cw == atomic_w1)
+        add_window_atomic<VEC, NSUB>(p, cw, colbase, lds, acc, cw != atomic_w0);
       else store_window<VEC, NSUB>(p, cw, colbase, acc);
       zero_acc<VEC, NSUB>(acc);
       gap0 = cw + 1;
@@ -615,7 +643,8 @@ __device__ __forceinline__ void walk_blocks(const SpmmParams &p, int b, int e, i
     cur = nxt;
     resolve_meta<true>(p, e, cur);
   }
-  if (cw == atomic_w0 || cw == atomic_w1) add_window_atomic<VEC, NSUB>(p, cw, colbase, lds, acc);
+  if (cw == atomic_w0 || cw == atomic_w1)
+    add_window_atomic<VEC, NSUB>(p, cw, colbase, lds, acc, cw != atomic_w0);
   else store_window<VEC, NSUB>(p, cw, colbase, acc);
   if (e == p.num_blocks) zero_windows<VEC, NSUB>(p, cw + 1, p.num_windows, colbase);
 }
@@ -663,6 +692,10 @@ __device__ __forceinline__ void run_unit(const SpmmParams &p, int item, int ct, 
     if (b >= e) return;
     const int w_first = cut_first ? wave_uniform(p.rowid[b]) : -1;
     const int w_last = cut_last ? wave_uniform(p.rowid[e - 1]) : -1;
+    if (p.parts && (cut_first || cut_last)) {
+      reinterpret_cast<int *>(lds)[wave_lds_floats<VEC, NSUB>()] = item;
+      wave_lds_sync();
+    }
     walk_blocks<PREC, VEC, NSUB>(p, b, e, colbase, lds, acc, has_vals, w_first, w_last);
   }
 }
@@ -687,7 +720,7 @@ template <int PREC, int VEC, int NSUB, bool BALANCED>
 __global__ __launch_bounds__(256) __attribute__((
     amdgpu_waves_per_eu(spmm_waves_per_eu<PREC, VEC, NSUB, BALANCED>()))) void spmm_kernel(SPMM_KERNEL_ARGS) {
   const SpmmParams p = SPMM_PARAMS_FROM_ARGS;
-  __shared__ __attribute__((aligned(16))) float s_tile[kWavesPerBlock][wave_lds_floats<VEC, NSUB>()];
+  __shared__ __attribute__((aligned(16))) float s_tile[kWavesPerBlock][wave_lds_words<VEC, NSUB>()];
   const int wave = wave_uniform(threadIdx.x >> 6);
   const int ct = blockIdx.x % p.n_ct;
   const int item = (blockIdx.x / p.n_ct) * kWavesPerBlock + wave;
@@ -701,7 +734,7 @@ template <int PREC, int VEC, int NSUB>
 __global__ __launch_bounds__(256) __attribute__((
     amdgpu_waves_per_eu(spmm_waves_per_eu<PREC, VEC, NSUB, true>()))) void spmm_kernel_persistent(SPMM_KERNEL_ARGS) {
   const SpmmParams p = SPMM_PARAMS_FROM_ARGS;
-  __shared__ __attribute__((aligned(16))) float s_tile[kWavesPerBlock][wave_lds_floats<VEC, NSUB>()];
+  __shared__ __attribute__((aligned(16))) float s_tile[kWavesPerBlock][wave_lds_words<VEC, NSUB>()];
   const int wave = wave_uniform(threadIdx.x >> 6);
   const int lane = lane_id();
   const int total = p.num_items * p.n_ct;
@@ -789,7 +822,8 @@ __device__ __forceinline__ int window_end_near(const int32_t *rowid, int c, int 
 __global__ __launch_bounds__(256) void k_balance_prep(const int32_t *rowid, int32_t T,
                                                       int32_t M, int32_t N,
                                                       int32_t chunk, int32_t num_items,
-                                                      float *C, int32_t *queue, int2 *runs) {
+                                                      float *C, int32_t *queue, int2 *runs,
+                                                      int32_t zero_cut) {
   if (queue && blockIdx.x == 0 && threadIdx.x == 0) *queue = 0;
   const int lane = lane_id();
   const int64_t v = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
@@ -828,8 +862,60 @@ __global__ __launch_bounds__(256) void k_balance_prep(const int32_t *rowid, int3
   }
   if (b >= c1) b = e = c1;
   if (lane == 0) runs[v] = make_int2(first_long ? ~b : b, last_long ? ~e : e);
+  if (!zero_cut) return;  // deterministic merge: k_balance_fixup writes cut windows whole
   if (first_long && ws0 == c0) zero_window_rows(C, w0, M, N);
   if (last_long && wsl >= c0 && !(wl == w0 && ws0 == c0)) zero_window_rows(C, wl, M, N);
+}
+
+// Deterministic merge of cut windows, after the SpMM kernel: wave (chunk c,
+// column tile ct) takes the one cut window that starts in chunk c, if any
+// (a cut window is longer than a chunk, so no chunk holds the start of two),
+// sums its partial tiles in chunk order -- chunk c's slot for the window's
+// role there, then slot 2 c' of every later chunk c' it reaches, where it is
+// the first window -- and stores the window's rows.  Tiles are row-major
+// 16 x tw floats (tw = the SpMM's column-tile width).
+__global__ __launch_bounds__(256) void k_balance_fixup(const int32_t *__restrict__ rowid,
+                                                       SpmmParams p, int32_t tw) {
+  const int wave = wave_uniform(threadIdx.x >> 6);
+  const int lane = lane_id();
+  const int ct = blockIdx.x % p.n_ct;
+  const int c = (blockIdx.x / p.n_ct) * kWavesPerBlock + wave;
+  if (c >= p.num_items) return;
+  const int2 r = p.runs[c];
+  const bool cut_first = r.x < 0;
+  const bool cut_last = r.y < 0;
+  if (!cut_first && !cut_last) return;
+  const int c0 = c * p.chunk;
+  const int b = cut_first ? ~r.x : r.x;
+  const int e = cut_last ? ~r.y : r.y;
+  int w = -1, role = 0;
+  if (cut_first) {  // the chunk's first window: cut, and starting at c0?
+    const int wf = wave_uniform(rowid[b]);
+    if (c0 == 0 || wave_uniform(rowid[c0 - 1]) != wf) w = wf;
+  }
+  if (w < 0 && cut_last) {  // the chunk's last window, starting inside the chunk?
+    const int wl = wave_uniform(rowid[e - 1]);
+    if (c0 == 0 || wave_uniform(rowid[c0 - 1]) != wl) {
+      w = wl;
+      role = 1;
+    }
+  }
+  if (w < 0) return;
+  const int we = wave_uniform(window_end_from(rowid, e - 1, p.num_blocks, w));
+  const int c_last = (we - 1) / p.chunk;
+  const int64_t tile = (int64_t)kBlkH * tw;
+  const int64_t slot_stride = (int64_t)p.n_ct * tile;
+  const float *first = p.parts + ((int64_t)2 * c + role) * slot_stride + ct * tile;
+  const float *later = p.parts + (int64_t)ct * tile;
+  const int64_t N = p.N;
+  for (int k = lane; k < kBlkH * tw; k += kWave) {
+    float sum = first[k];
+#pragma unroll 8
+    for (int cc = c + 1; cc <= c_last; ++cc) sum += later[(int64_t)2 * cc * slot_stride + k];
+    const int64_t row = (int64_t)w * kBlkH + k / tw;
+    const int col = ct * tw + k % tw;
+    if (row < p.M && col < p.N) __builtin_nontemporal_store(sum, p.C + row * N + col);
+  }
 }
 
 // ------------------------------------------------------------ host side
@@ -965,10 +1051,25 @@ int default_chunk_blocks(int32_t num_blocks) {
 }
 
 // Balanced workspace: the persistent work counter, then one int2 run per
-// chunk (at most one chunk per block).
+// chunk; with the deterministic merge, then two partial-tile slots per chunk
+// of 16 rows x N floats (N rounded up to 128, a multiple of every
+// column-tile width).  Without a chunk size, one run per block covers every
+// chunk size.
 constexpr size_t kQueueBytes = 256;
 size_t balanced_workspace_bytes(int32_t num_blocks) {
   return kQueueBytes + sizeof(int2) * (size_t)std::max<int32_t>(num_blocks, 0);
+}
+int64_t num_chunks(int32_t num_blocks, int32_t chunk) {
+  return ceil_div(std::max<int32_t>(num_blocks, 0), std::max<int32_t>(chunk, 1));
+}
+size_t balanced_parts_offset(int32_t num_blocks, int32_t chunk) {
+  return align_up(kQueueBytes + sizeof(int2) * (size_t)num_chunks(num_blocks, chunk), 256);
+}
+size_t balanced_workspace_bytes_for(int32_t num_blocks, int32_t n, int32_t chunk, bool det) {
+  if (!det) return kQueueBytes + sizeof(int2) * (size_t)num_chunks(num_blocks, chunk);
+  const int64_t npad = ceil_div(std::max<int32_t>(n, 1), 128) * 128;
+  return balanced_parts_offset(num_blocks, chunk) +
+         (size_t)(2 * num_chunks(num_blocks, chunk)) * kBlkH * (size_t)npad * sizeof(float);
 }
 
 __global__ __launch_bounds__(256) void k_zero(float *c, int64_t count) {
@@ -1033,7 +1134,8 @@ __global__ __launch_bounds__(256) void spmm_tcgnn_kernel(
       const int gg = (c >> 3) - k0;
       if (gg >= 0 && gg < kGroup) {
         const int slot = c & (kBlkW - 1);
-        tile[gg * (kBlkH * kBlkW) + (e2r[e] & (kBlkH - 1)) * kBlkW + slot] = vals ? vals[e] : 1.0f;
+        atomicAdd(&tile[gg * (kBlkH * kBlkW) + (e2r[e] & (kBlkH - 1)) * kBlkW + slot],
+                  vals ? vals[e] : 1.0f);  // repeated (row, column) pairs add up
         atox[gg * kBlkW + slot] = col[e];
       }
     }
@@ -1177,6 +1279,21 @@ dtc_status_t dtc_spmm_balanced_workspace_size(int32_t num_blocks, size_t *bytes)
   return DTC_SUCCESS;
 }
 
+dtc_status_t dtc_spmm_balanced_workspace_size_ex(int32_t num_blocks, int32_t n,
+                                                 const dtc_spmm_options_t *options, size_t *bytes) {
+  clear_error();
+  if (!bytes) return fail(DTC_ERROR_INVALID_VALUE, "bytes is NULL");
+  if (num_blocks < 0) return fail(DTC_ERROR_INVALID_VALUE, "num_blocks must be >= 0");
+  if (n < 0) return fail(DTC_ERROR_INVALID_VALUE, "N must be >= 0");
+  if (options && options->chunk_blocks < 0)
+    return fail(DTC_ERROR_INVALID_VALUE, "chunk_blocks must be >= 0");
+  const bool det = options && options->deterministic;
+  const int chunk = (options && options->chunk_blocks > 0) ? options->chunk_blocks
+                                                           : default_chunk_blocks(num_blocks);
+  *bytes = balanced_workspace_bytes_for(num_blocks, n, chunk, det);
+  return DTC_SUCCESS;
+}
+
 dtc_status_t dtc_spmm_balanced(const int32_t *tcblock_rowid, const uint8_t *tcblocktile_id,
                                const int32_t *tcblock_offset, const int32_t *sparse_atox,
                                const float *tcblock_values, int32_t num_blocks, int32_t nnz,
@@ -1196,11 +1313,16 @@ dtc_status_t dtc_spmm_balanced(const int32_t *tcblock_rowid, const uint8_t *tcbl
   const int chunk = (options && options->chunk_blocks > 0) ? options->chunk_blocks
                                                            : default_chunk_blocks(num_blocks);
   const bool persistent = options && options->persistent;
+  if (options && (options->deterministic < 0 || options->deterministic > 1))
+    return fail(DTC_ERROR_INVALID_VALUE, "deterministic must be 0 or 1");
+  const bool det = options && options->deterministic;
   const int32_t W = (int32_t)ceil_div(num_nodes, kBlkH);
   if (W == 0) return DTC_SUCCESS;
-  if (num_blocks > 0 && (!workspace || workspace_bytes < balanced_workspace_bytes(num_blocks)))
+  const size_t need = balanced_workspace_bytes_for(num_blocks, n, chunk, det);
+  if (num_blocks > 0 && (!workspace || workspace_bytes < need))
     return fail(DTC_ERROR_WORKSPACE,
-                "balanced SpMM needs the workspace that dtc_spmm_balanced_workspace_size reports");
+                "balanced SpMM needs the workspace that dtc_spmm_balanced_workspace_size_ex reports (" +
+                    std::to_string(need) + " bytes)");
   if (b_rows == 0 || num_blocks == 0) return zero_output(c, num_nodes, n, stream);
   SpmmParams p{};
   p.rowid = tcblock_rowid;
@@ -1222,9 +1344,12 @@ dtc_status_t dtc_spmm_balanced(const int32_t *tcblock_rowid, const uint8_t *tcbl
   p.queue = persistent ? static_cast<int32_t *>(workspace) : nullptr;
   int2 *runs = reinterpret_cast<int2 *>(static_cast<char *>(workspace) + kQueueBytes);
   p.runs = runs;
+  p.parts = det ? reinterpret_cast<float *>(static_cast<char *>(workspace) +
+                                            balanced_parts_offset(num_blocks, chunk))
+                : nullptr;
   const int64_t prep_waves = p.num_items;
   k_balance_prep<<<(unsigned)ceil_div(prep_waves, kWavesPerBlock), 256, 0, stream>>>(
-      tcblock_rowid, num_blocks, num_nodes, n, chunk, p.num_items, c, p.queue, runs);
+      tcblock_rowid, num_blocks, num_nodes, n, chunk, p.num_items, c, p.queue, runs, det ? 0 : 1);
   DTC_LAUNCH_CHECK("balance_prep");
   if (!persistent) {
     KernelFn fn = pick_kernel(var, 1);
@@ -1244,6 +1369,11 @@ dtc_status_t dtc_spmm_balanced(const int32_t *tcblock_rowid, const uint8_t *tcbl
                      p.atox, p.vals, p.B, p.C, p);
   }
   DTC_LAUNCH_CHECK("spmm_balanced");
+  if (det) {
+    const int64_t grid = (int64_t)p.n_ct * ceil_div(p.num_items, kWavesPerBlock);
+    k_balance_fixup<<<(unsigned)grid, 256, 0, stream>>>(tcblock_rowid, p, 16 * var.vec * var.nsub);
+    DTC_LAUNCH_CHECK("balance_fixup");
+  }
   return DTC_SUCCESS;
 }
 

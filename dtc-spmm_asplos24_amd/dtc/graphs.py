@@ -135,35 +135,56 @@ def sbm_hubs(num_rows: int = 232965, communities: int = 50, zipf_s: float = 1.1,
              avg_per_row: float = 490.0, cross_frac: float = 0.05, hubs: int = 200,
              hub_min: int = 20000, hub_max: int = 50000, seed: int = 3,
              values: str | None = "uniform01") -> CSR:
-    """Stochastic block model with Zipf(zipf_s) community sizes (contiguous
-    node ids per community), ~avg_per_row entries per row of which
-    ``cross_frac`` leave the community, plus ``hubs`` rows with
-    U[hub_min, hub_max] extra uniform columns.  Values U[0, 1] float32."""
+    """Stochastic block model with hub rows.
+
+    Communities take contiguous node ids with sizes proportional to
+    k**-zipf_s (k = 1..communities; every community holds at least one row,
+    the first takes the rounding remainder).  A pair (i, j) with j in i's
+    community is an entry with probability p_in(c) = avg_per_row *
+    (1 - cross_frac) / size_c (capped below 1), and any pair with probability
+    p_out = avg_per_row * cross_frac / num_rows, independently -- so a row
+    holds about avg_per_row entries, a fraction cross_frac of them drawn
+    over all columns.  Each pair set is sampled by drawing Poisson(lam)
+    columns with replacement, lam = -size * ln(1 - p), and dropping repeats:
+    a column's draw count is then Poisson(lam / size) and it is present
+    with probability 1 - exp(-lam / size) = p, independently of the others.
+    ``hubs`` distinct rows get U[hub_min, hub_max] further distinct uniform
+    columns.  Values U[0, 1] float32."""
     rng = np.random.Generator(np.random.PCG64(seed))
-    w = 1.0 / np.arange(1, communities + 1) ** zipf_s
-    sizes = np.maximum(1, np.floor(w / w.sum() * num_rows)).astype(np.int64)
-    sizes[0] += num_rows - sizes.sum()
-    starts = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    bounds = sbm_community_starts(num_rows, communities, zipf_s)
+    sizes, starts = np.diff(bounds), bounds[:-1]
     comm_of = np.repeat(np.arange(communities), sizes)
-    n_in = rng.poisson(avg_per_row * (1 - cross_frac), num_rows)
-    n_out = rng.poisson(avg_per_row * cross_frac, num_rows)
-    n_in = np.minimum(n_in, sizes[comm_of])
+    p_in = np.minimum(avg_per_row * (1.0 - cross_frac) / sizes, 0.999)
+    lam_in = -sizes * np.log1p(-p_in)
+    p_out = min(avg_per_row * cross_frac / num_rows, 0.999)
+    lam_out = -num_rows * np.log1p(-p_out)
+    n_in = rng.poisson(lam_in[comm_of])
+    n_out = rng.poisson(lam_out, num_rows)
     rows_in = np.repeat(np.arange(num_rows), n_in)
     c_in = comm_of[rows_in]
     cols_in = starts[c_in] + (rng.random(rows_in.shape[0]) * sizes[c_in]).astype(np.int64)
     rows_out = np.repeat(np.arange(num_rows), n_out)
     cols_out = rng.integers(0, num_rows, rows_out.shape[0])
     hub_rows = rng.choice(num_rows, min(hubs, num_rows), replace=False)
-    hub_n = rng.integers(hub_min, hub_max + 1, hub_rows.shape[0])
-    hub_n = np.minimum(hub_n, num_rows)
+    hub_n = np.minimum(rng.integers(hub_min, hub_max + 1, hub_rows.shape[0]), num_rows)
     rows_h = np.repeat(hub_rows, hub_n)
-    cols_h = rng.integers(0, num_rows, rows_h.shape[0])
+    cols_h = np.concatenate([rng.choice(num_rows, k, replace=False) for k in hub_n]
+                            or [np.zeros(0, np.int64)])
     rows = np.concatenate([rows_in, rows_out, rows_h])
     cols = np.concatenate([cols_in, cols_out, cols_h])
     csr = _dedup_pairs(rows, cols, num_rows, num_rows)
     if values == "uniform01":
         csr.values = rng.random(csr.nnz).astype(np.float32)
     return csr
+
+
+def sbm_community_starts(num_rows: int = 232965, communities: int = 50,
+                         zipf_s: float = 1.1) -> np.ndarray:
+    """First row of every community of ``sbm_hubs`` (and num_rows at the end)."""
+    w = 1.0 / np.arange(1, communities + 1) ** zipf_s
+    sizes = np.maximum(1, np.floor(w / w.sum() * num_rows)).astype(np.int64)
+    sizes[0] += num_rows - sizes.sum()
+    return np.concatenate([[0], np.cumsum(sizes)])
 
 
 def banded(num_rows: int, half_width: int, fill: float = 0.5, seed: int = 0) -> CSR:

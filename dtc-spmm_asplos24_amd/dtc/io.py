@@ -92,6 +92,5 @@ def load_metcf(path: str, device="cuda"):
     if A.values is not None:
         from . import load_extension
         A.values_tc = load_extension().gather(A.values, A.tc_edge)
-    A._transpose = None
-    A._perm_t = None
+    A._reset_caches()
     return A

@@ -47,6 +47,11 @@ class DTCMatrix:
         (self.row_window_offset, self.tcblock_rowid, self.tcblocktile_id, self.tcblock_offset,
          self.sparse_atox, self.block_count, self.tc_edge) = out
         self.values_tc = None if self.values is None else ext.gather(self.values, self.tc_edge)
+        self._reset_caches()
+
+    def _reset_caches(self) -> None:
+        """Clears what is derived lazily from the ME-TCF arrays (also used
+        when a matrix is rebuilt from a cache file without ``__init__``)."""
         self._transpose = None
         self._perm_t = None
         self._max_window_blocks = None
@@ -78,10 +83,13 @@ class DTCMatrix:
     def spmm(self, B: torch.Tensor, schedule: str = "auto", precision: str = "fp32",
              values_tc: torch.Tensor | None = None, use_values: bool = True, vec: int = 0,
              subtiles: int = 0, chunk_blocks: int = 0, persistent: bool = False,
-             out: torch.Tensor | None = None) -> torch.Tensor:
+             out: torch.Tensor | None = None, deterministic: bool = False) -> torch.Tensor:
         """C = A * B.  schedule "auto" runs the unbalanced schedule when no row
         window is longer than ``dtc.selector.AUTO_WINDOW_BLOCKS`` TC blocks
-        and the balanced one otherwise; "windows" / "balanced" force one."""
+        and the balanced one otherwise; "windows" / "balanced" force one.
+        ``deterministic`` (balanced schedule) merges windows cut across chunks
+        in a fixed order, so repeated calls are bitwise identical; the
+        unbalanced schedule is deterministic as it is."""
         ext = load_extension()
         if schedule == "auto":
             from .selector import default_schedule
@@ -91,7 +99,8 @@ class DTCMatrix:
         return ext.spmm(B.contiguous(), idx, self.tcblocktile_id, self.tcblock_offset,
                         self.sparse_atox, self.num_rows, schedule=schedule, values=vals,
                         precision=precision, vec=vec, subtiles=subtiles,
-                        chunk_blocks=chunk_blocks, persistent=persistent, out=out)
+                        chunk_blocks=chunk_blocks, persistent=persistent, out=out,
+                        deterministic=deterministic)
 
     # ------------------------------------------------------------- transpose
     def transpose(self) -> "DTCMatrix":
@@ -137,6 +146,14 @@ class _SpMMFunction(torch.autograd.Function):
             vals_t_csr = ext.gather(values.contiguous(), A.perm_t)
             vals_t_tc = ext.gather(vals_t_csr, At.tc_edge)
             d_B = At.spmm(dC, schedule=ctx.schedule, precision=ctx.precision, values_tc=vals_t_tc)
+            # A^T has num_cols rows; B may have more (rows no column of A
+            # reaches get zero gradient) or fewer (columns of A past B's rows
+            # read as zero in the forward, so their rows of A^T dC are dropped).
+            kb = B.shape[0]
+            if d_B.shape[0] < kb:
+                d_B = torch.cat([d_B, d_B.new_zeros((kb - d_B.shape[0], d_B.shape[1]))])
+            elif d_B.shape[0] > kb:
+                d_B = d_B[:kb].contiguous()
         return d_values, d_B, None, None, None
 
 

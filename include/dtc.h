@@ -63,6 +63,11 @@ typedef struct dtc_spmm_options {
                            power of two in [32, 512] at most num_blocks / 8192) */
   int32_t persistent;   /* balanced schedule: 1 = persistent waves pulling chunks from a queue */
   int32_t b_dtype;      /* dtc_b_dtype_t */
+  int32_t deterministic; /* balanced schedule: 1 = windows cut across chunks are merged in a
+                            fixed order (partial tiles in the workspace, then one ordered sum
+                            per window) instead of float32 atomics, so repeated calls give
+                            bitwise identical results; needs the larger workspace that
+                            dtc_spmm_balanced_workspace_size_ex reports */
 } dtc_spmm_options_t;
 
 /* ------------------------------------------------------------------ misc */
@@ -161,14 +166,24 @@ dtc_status_t dtc_spmm_windows(const int32_t *row_window_offset, const uint8_t *t
  * Balanced schedule: work chunks of options->chunk_blocks consecutive TC
  * blocks, each split at row-window boundaries; windows that span several
  * chunks are merged with float32 atomic adds into rows this call zeroes
- * first.  Replaces run_DTCSpMM_balance /
+ * first (or, with options->deterministic, summed in chunk order from
+ * partial tiles in the workspace).  Replaces run_DTCSpMM_balance /
  * spmm_balance_forward_cuda_ptx_unit8_prefetch (DTCSpMM.cpp:163-178,
  * DTCSpMM_kernel.cu:1118-1199).  Every output element is written.  The
- * workspace (required when num_blocks > 0; 256 + 8 * num_blocks bytes) holds
- * each chunk's block run, worked out by a preparation kernel, and the work
- * counter of the persistent variant.
+ * workspace (required when num_blocks > 0) holds each chunk's block run,
+ * worked out by a preparation kernel, the work counter of the persistent
+ * variant and, with options->deterministic, the partial tiles; its size is
+ * dtc_spmm_balanced_workspace_size_ex for these options and N (exactly
+ * 256 + 8 * ceil(num_blocks / chunk) bytes without the deterministic merge),
+ * or dtc_spmm_balanced_workspace_size (256 + 8 * num_blocks bytes, enough for
+ * any chunk size without the deterministic merge).
  */
 dtc_status_t dtc_spmm_balanced_workspace_size(int32_t num_blocks, size_t *bytes);
+/* Workspace bytes for the given options and N (NULL options: the defaults):
+ * 256 + 8 * ceil(num_blocks / chunk), plus with options->deterministic two
+ * partial-tile slots of 16 x N float32 (N rounded up to 128) per chunk. */
+dtc_status_t dtc_spmm_balanced_workspace_size_ex(int32_t num_blocks, int32_t n,
+                                                 const dtc_spmm_options_t *options, size_t *bytes);
 dtc_status_t dtc_spmm_balanced(const int32_t *tcblock_rowid, const uint8_t *tcblocktile_id,
                                const int32_t *tcblock_offset, const int32_t *sparse_atox,
                                const float *tcblock_values, int32_t num_blocks, int32_t nnz,

@@ -277,9 +277,12 @@ void oracle_spmm_csr_f64(int32_t M, const int32_t *rowptr, const int32_t *col,
 /* C = A * B in float64 with A decoded from ME-TCF: for every block b of
  * window w and every stored element p of b, tile id t gives row
  * w*blk_h + t / blk_w and column sparseAToX[b*blk_w + t % blk_w]
- * (DTCSpMM_kernel.cu:2716-2722 builds the same dense tile).  A later
- * element with the same tile id overwrites an earlier one, as the
- * shared-memory tile store does.  tcValues NULL means all ones. */
+ * (DTCSpMM_kernel.cu:2716-2722 builds the same dense tile).  Elements with
+ * the same tile id (a repeated (row, column) pair in the CSR) are added: the
+ * reference's shared-memory store lets one of the racing writes win, and its
+ * loader never produces such pairs because it sums duplicates
+ * (scripts/DTCSpMM/dataset.py:36-44); adding them makes the ME-TCF product
+ * equal to the CSR product for every input.  tcValues NULL means all ones. */
 int oracle_spmm_metcf_f64(int32_t M, int32_t num_windows,
                           const int32_t *rowWindowOffset,
                           const uint8_t *tileId, const int32_t *tcblockOffset,
@@ -302,7 +305,7 @@ int oracle_spmm_metcf_f64(int32_t M, int32_t num_windows,
           err = ORACLE_ERR_INCONSISTENT;
           continue;
         }
-        tile[t] = tcValues ? (double)tcValues[p] : 1.0;
+        tile[t] += tcValues ? (double)tcValues[p] : 1.0;
       }
       for (int r = 0; r < blk_h; ++r) {
         const int row = w * blk_h + r;

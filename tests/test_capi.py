@@ -49,7 +49,8 @@ def test_status_strings(lib):
 
 class Opts(ctypes.Structure):  # dtc_spmm_options_t
     _fields_ = [("precision", ctypes.c_int32), ("vec", ctypes.c_int32), ("subtiles", ctypes.c_int32),
-                ("chunk_blocks", ctypes.c_int32), ("persistent", ctypes.c_int32), ("b_dtype", ctypes.c_int32)]
+                ("chunk_blocks", ctypes.c_int32), ("persistent", ctypes.c_int32), ("b_dtype", ctypes.c_int32),
+                ("deterministic", ctypes.c_int32)]
 
 
 class TcaOpts(ctypes.Structure):  # dtc_tca_options_t
@@ -59,7 +60,9 @@ class TcaOpts(ctypes.Structure):  # dtc_tca_options_t
 
 
 def test_struct_sizes_match_header():
-    assert ctypes.sizeof(Opts) == 24 and ctypes.sizeof(TcaOpts) == 32
+    assert ctypes.sizeof(Opts) == 28 and ctypes.sizeof(TcaOpts) == 32
+    from dtc.capi import SpmmOptions
+    assert [f[0] for f in SpmmOptions._fields_] == [f[0] for f in Opts._fields_]
 
 
 def test_reorder_and_sddmm_argument_errors(lib):
@@ -117,14 +120,33 @@ def test_host_side_argument_errors(lib):
     assert lib.dtc_spmm_balanced_workspace_size(4, ctypes.byref(size)) == 0
     assert size.value == 256 + 8 * 4
     assert lib.dtc_spmm_balanced_workspace_size(-1, ctypes.byref(size)) == 1
+    # exact sizes for a given chunk: one int2 run per chunk, and with the
+    # deterministic merge two 16 x 128 float32 slots per chunk (N = 8 -> 128)
+    o = Opts(0, 0, 0, 2, 0, 0, 0)
+    assert lib.dtc_spmm_balanced_workspace_size_ex(4, 8, ctypes.byref(o), ctypes.byref(size)) == 0
+    assert size.value == 256 + 8 * 2
+    o.deterministic = 1
+    assert lib.dtc_spmm_balanced_workspace_size_ex(4, 8, ctypes.byref(o), ctypes.byref(size)) == 0
+    assert size.value == 512 + 2 * 2 * 16 * 128 * 4
+    assert lib.dtc_spmm_balanced_workspace_size_ex(4, 200, ctypes.byref(o), ctypes.byref(size)) == 0
+    assert size.value == 512 + 2 * 2 * 16 * 256 * 4
+    assert lib.dtc_spmm_balanced_workspace_size_ex(4, 8, None, ctypes.byref(size)) == 0
+    assert size.value == 256 + 8 * 1  # default chunk for 4 blocks: 32
     for pers in (0, 1):
-        o = Opts(0, 0, 0, 64, pers)
-        st = lib.dtc_spmm_balanced(dummy, dummy, dummy, dummy, None, 4, 8, 16, dummy, 16, 8, dummy,
-                                   ctypes.byref(o), None, 0, None)
-        assert st == 3
-        st = lib.dtc_spmm_balanced(dummy, dummy, dummy, dummy, None, 4, 8, 16, dummy, 16, 8, dummy,
-                                   ctypes.byref(o), dummy, 256 + 8 * 3, None)
-        assert st == 3 and b"workspace" in lib.dtc_last_error()
+        for det in (0, 1):
+            o = Opts(0, 0, 0, 2, pers, 0, det)
+            need = ctypes.c_size_t()
+            assert lib.dtc_spmm_balanced_workspace_size_ex(4, 8, ctypes.byref(o), ctypes.byref(need)) == 0
+            st = lib.dtc_spmm_balanced(dummy, dummy, dummy, dummy, None, 4, 8, 16, dummy, 16, 8, dummy,
+                                       ctypes.byref(o), None, 0, None)
+            assert st == 3
+            st = lib.dtc_spmm_balanced(dummy, dummy, dummy, dummy, None, 4, 8, 16, dummy, 16, 8, dummy,
+                                       ctypes.byref(o), dummy, need.value - 1, None)
+            assert st == 3 and b"workspace" in lib.dtc_last_error()
+    o = Opts(0, 0, 0, 2, 0, 0, 2)  # deterministic must be 0 or 1
+    st = lib.dtc_spmm_balanced(dummy, dummy, dummy, dummy, None, 4, 8, 16, dummy, 16, 8, dummy,
+                               ctypes.byref(o), dummy, 1 << 20, None)
+    assert st == 1
 
 
 @pytest.fixture(scope="module")

@@ -134,3 +134,55 @@ def test_gradients_rmat_2048_finite_differences(ext):
         an_B = float((gB * dB).sum())
         assert abs(fd_v - an_v) <= 1e-3 * max(1.0, abs(fd_v)), (trial, fd_v, an_v)
         assert abs(fd_B - an_B) <= 1e-3 * max(1.0, abs(fd_B)), (trial, fd_B, an_B)
+
+
+@pytest.mark.parametrize("schedule", ["windows", "balanced"])
+def test_autograd_b_rows_differ_from_num_cols(ext, schedule):
+    """num_cols inferred as max column + 1 (fewer rows than B), and a B with
+    fewer rows than num_cols (columns past B's rows read as zero): dB always
+    has B's shape, rows past the last column of A get zero gradient."""
+    from dtc.matrix import DTCMatrix, spmm_autograd
+    rng = np.random.default_rng(43)
+    M, K, N = 200, 300, 24
+    rowptr, col = random_csr(rng, M, 250, 5.0, empty_frac=0.1)  # columns < 250 <= K
+    vals = rng.uniform(-1, 1, col.shape[0]).astype(np.float32)
+    dC = rng.standard_normal((M, N)).astype(np.float32)
+    for num_cols, b_rows in [(None, K), (K, 180)]:
+        say("autograd b_rows", schedule, num_cols, b_rows)
+        B = rng.standard_normal((b_rows, N)).astype(np.float32)
+        A = DTCMatrix(rowptr, col, None, num_rows=M, num_cols=num_cols)
+        v = torch.from_numpy(vals).cuda().requires_grad_(True)
+        Bt = torch.from_numpy(B).cuda().requires_grad_(True)
+        spmm_autograd(v, Bt, A, schedule=schedule).backward(torch.from_numpy(dC).cuda())
+        assert Bt.grad.shape == Bt.shape
+        keep = col < b_rows  # entries whose column exists in B
+        kc = int(max(b_rows, col.max() + 1))
+        Bpad = np.zeros((kc, N), np.float32)
+        Bpad[:b_rows] = B
+        rp = np.concatenate([[0], np.cumsum([keep[rowptr[r]:rowptr[r + 1]].sum() for r in range(M)])])
+        gv, gB, _ = _grads_f64(rp.astype(np.int32), col[keep], vals[keep], Bpad, dC, M, kc)
+        assert np.allclose(Bt.grad.cpu().numpy(), gB[:b_rows], rtol=1e-4, atol=1e-5)
+        assert np.allclose(v.grad.cpu().numpy()[keep], gv, rtol=1e-4, atol=1e-4)
+
+
+@pytest.mark.parametrize("schedule", ["windows", "balanced"])
+def test_autograd_duplicate_entries(ext, schedule):
+    """Repeated (row, column) pairs: the forward adds them, as the CSR
+    product does, and each copy gets its own gradient."""
+    from dtc.matrix import DTCMatrix, spmm_autograd
+    rng = np.random.default_rng(44)
+    M, N = 300, 40
+    rowptr, col = random_csr(rng, M, M, 6.0, sorted_rows=False, dup_frac=0.6)
+    vals = rng.uniform(-1, 1, col.shape[0]).astype(np.float32)
+    B = rng.standard_normal((M, N)).astype(np.float32)
+    dC = rng.standard_normal((M, N)).astype(np.float32)
+    say("autograd duplicates", schedule, col.shape[0])
+    A = DTCMatrix(rowptr, col, None, num_rows=M, num_cols=M)
+    v = torch.from_numpy(vals).cuda().requires_grad_(True)
+    Bt = torch.from_numpy(B).cuda().requires_grad_(True)
+    C = spmm_autograd(v, Bt, A, schedule=schedule)
+    C.backward(torch.from_numpy(dC).cuda())
+    gv, gB, C64 = _grads_f64(rowptr, col, vals, B, dC, M, M)  # torch.sparse coalesces by adding
+    assert np.allclose(C.detach().cpu().numpy(), C64, rtol=1e-4, atol=1e-5)
+    assert np.allclose(v.grad.cpu().numpy(), gv, rtol=1e-4, atol=1e-4)
+    assert np.allclose(Bt.grad.cpu().numpy(), gB, rtol=1e-4, atol=1e-5)

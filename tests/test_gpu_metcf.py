@@ -151,3 +151,16 @@ def test_metcf_cache_roundtrip(ext, tmp_path):
     B = torch.randn((a.num_rows, 32), device="cuda")
     assert torch.equal(A.spmm(B, schedule="windows"), A2.spmm(B, schedule="windows"))
     assert A2.block_count == A.block_count and torch.equal(A2.values_tc, A.values_tc)
+    # the default schedule ("auto") and the differentiable product work on a
+    # matrix read back from the cache, with the same results
+    assert A2.max_window_blocks == A.max_window_blocks
+    assert torch.equal(A.spmm(B), A2.spmm(B))
+    from dtc.matrix import spmm_autograd
+    v = torch.from_numpy(a.values).cuda()
+    grads = []
+    for M_ in (A, A2):
+        vv = v.clone().requires_grad_(True)
+        Bb = B.clone().requires_grad_(True)
+        spmm_autograd(vv, Bb, M_).square().sum().backward()
+        grads.append((vv.grad, Bb.grad))
+    assert torch.equal(grads[0][0], grads[1][0]) and torch.equal(grads[0][1], grads[1][1])

@@ -270,6 +270,42 @@ def test_repeated_launches_identical(ext, ragged):
         assert componentwise(out.cpu().numpy(), C64, Cabs) <= TOL["fp32"]
 
 
+@pytest.mark.parametrize("persistent", [False, True])
+def test_deterministic_merge_bitwise(ext, ragged, rmat12, persistent):
+    """The deterministic merge: windows cut across chunks are summed in chunk
+    order from partial tiles, so repeated calls are bitwise identical (output
+    pre-filled with NaN: every element must be written), for chunk sizes that
+    cut windows into many pieces, every tile shape and bf16 B."""
+    for case in (ragged, rmat12):
+        rng = np.random.default_rng(90)
+        for N in (1, 17, 64, 130):
+            Bn = rng.standard_normal((case.M, N)).astype(np.float32)
+            B = torch.from_numpy(Bn).cuda()
+            C64, Cabs = case.ref(Bn)
+            for chunk in (1, 3, 8, 64):
+                for vec, sub in [(0, 0), (1, 1), (2, 1), (4, 1), (4, 2)]:  # narrowed to what N allows
+                    say("deterministic", case.name, persistent, N, chunk, vec, sub)
+                    first = None
+                    for _ in range(3):
+                        out = torch.full((case.M, N), float("nan"), device="cuda")
+                        case.run(ext, B, "balanced", "fp32", chunk_blocks=chunk, vec=vec, subtiles=sub,
+                                 persistent=persistent, deterministic=True, out=out)
+                        if first is None:
+                            first = out.clone()
+                            assert componentwise(out.cpu().numpy(), C64, Cabs) <= TOL["fp32"], (N, chunk, vec)
+                        else:
+                            assert torch.equal(out, first), (case.name, N, chunk, vec, sub)
+    # bf16 B (gathered as stored) through the same merge
+    Bb = torch.randn((rmat12.M, 256), device="cuda").to(torch.bfloat16)
+    Bn = Bb.float().cpu().numpy()
+    C64, Cabs = rmat12.ref(Bn)
+    say("deterministic bf16 B", persistent)
+    outs = [rmat12.run(ext, Bb, "balanced", "fp32", chunk_blocks=4, persistent=persistent, deterministic=True)
+            for _ in range(3)]
+    assert componentwise(outs[0].cpu().numpy(), C64, Cabs) <= TOL["fp32"]
+    assert torch.equal(outs[0], outs[1]) and torch.equal(outs[0], outs[2])
+
+
 def test_reference_call_sequence_degree(ext):
     """The sequence of scripts/DTCSpMM/run_DTC_SpMM.py:45-79 with X = ones:
     every plan returns C[r, :] = deg(r) exactly."""
@@ -319,3 +355,34 @@ def test_b_row_guard(ext):
     say("spmm B row guard")
     C = case.run(ext, B, "windows", "fp32")
     assert torch.equal(C[:, 0].cpu(), torch.tensor([1.0, 1.0]))
+
+
+@pytest.mark.parametrize("schedule", ["windows", "balanced"])
+def test_duplicate_entries_add(ext, schedule):
+    """A CSR with repeated (row, column) pairs, values and A = 1: the stored
+    elements that share a tile id are added, so the ME-TCF product equals the
+    CSR product (and the degree answer counts every copy)."""
+    rng = np.random.default_rng(77)
+    M = 1500
+    rowptr, col = random_csr(rng, M, M, 10.0, sorted_rows=False, empty_frac=0.1, dup_frac=0.7)
+    # a row holding one column many times, and a hub with repeats
+    rows = [list(col[rowptr[r]:rowptr[r + 1]]) for r in range(M)]
+    rows[4] = [9] * 70
+    rows[700] = list(rng.integers(0, M, 600))
+    rowptr = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int32)
+    col = np.concatenate(rows).astype(np.int32)
+    vals = rng.uniform(-1, 1, col.shape[0]).astype(np.float32)
+    case = Case(ext, rowptr, col, M, vals, "duplicates")
+    Bn = rng.standard_normal((M, 64)).astype(np.float32)
+    B = torch.from_numpy(Bn).cuda()
+    for use_vals in (True, False):
+        C64, Cabs = case.ref(Bn, use_vals)
+        for precision in ("fp32", "bf16x6"):
+            for chunk in (1, 4, 64):
+                say("spmm duplicates", schedule, use_vals, precision, chunk)
+                C = case.run(ext, B, schedule, precision, use_vals=use_vals, chunk_blocks=chunk).cpu().numpy()
+                assert componentwise(C, C64, Cabs) <= TOL[precision], (use_vals, precision, chunk)
+    ones = torch.ones((M, 16), device="cuda")
+    deg = torch.from_numpy(np.diff(rowptr).astype(np.float32)).cuda()
+    C = case.run(ext, ones, schedule, "fp32", use_vals=False)
+    assert torch.equal(C, deg[:, None].expand_as(C))

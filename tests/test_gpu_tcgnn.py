@@ -32,7 +32,8 @@ def phase1(ext, rowptr, col, M):
 def test_tcgnn_vs_float64(ext, N):
     rng = np.random.default_rng(N)
     M, K = 1200, 900
-    rowptr, col = random_csr(rng, M, K, 10.0, empty_frac=0.2)
+    # odd N: repeated (row, column) pairs, whose values add up
+    rowptr, col = random_csr(rng, M, K, 10.0, empty_frac=0.2, dup_frac=0.5 if N % 2 else 0.0)
     vals = rng.uniform(-1, 1, col.shape[0]).astype(np.float32)
     B = rng.standard_normal((K, N)).astype(np.float32)
     say("tcgnn", N)

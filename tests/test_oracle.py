@@ -116,14 +116,20 @@ def test_spmm_csr_vs_scipy():
     assert np.allclose(Cabs, Aabs @ np.abs(B).astype(np.float64), rtol=1e-12, atol=1e-12)
 
 
-def test_spmm_metcf_matches_csr():
+@pytest.mark.parametrize("dup_frac", [0.0, 0.7])
+def test_spmm_metcf_matches_csr(dup_frac):
+    """The ME-TCF products equal the CSR product, also when (row, column)
+    pairs repeat (their elements share a tile id and are added)."""
     rng = np.random.default_rng(4)
     M, N = 333, 24
-    rowptr, col = random_csr(rng, M, M, 9.0, sorted_rows=False, empty_frac=0.1)
+    rowptr, col = random_csr(rng, M, M, 9.0, sorted_rows=False, empty_frac=0.1, dup_frac=dup_frac)
     vals = rng.uniform(-1, 1, col.shape[0]).astype(np.float32)
     B = rng.standard_normal((M, N)).astype(np.float32)
     m = oracle.metcf(rowptr, col, M, values=vals)
     C = oracle.spmm_csr_f64(rowptr, col, vals, B, M)
+    A = sp.coo_matrix((vals.astype(np.float64), (np.repeat(np.arange(M), np.diff(rowptr)), col)),
+                      shape=(M, M)).tocsr()  # scipy sums duplicates
+    assert np.allclose(C, A @ B.astype(np.float64), rtol=1e-12, atol=1e-12)
     C64 = oracle.spmm_metcf_f64(m, B, M, tc_values=m["TCblockValues"])
     assert np.allclose(C64, C, rtol=1e-12, atol=1e-12)
     C32 = oracle.spmm_metcf_f32(m, B, M, tc_values=m["TCblockValues"])
