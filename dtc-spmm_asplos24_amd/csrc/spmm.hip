@@ -25,10 +25,18 @@
 // group is a permutation shared by A and B, so every mapping sums the same
 // products.
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <type_traits>
 
 #include "dtc_internal.h"
+
+// Workspace of dtc_spmm_balanced_noalloc, for callers that cannot allocate:
+// a fixed device array, one copy per GPU, held until the process ends.  Host
+// code never uses this name as an address (on the host it is a stand-in
+// symbol); the device address comes from hipGetSymbolAddress, per device.
+constexpr size_t kNoAllocScratchBytes = (size_t)8 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
 namespace dtc {
 namespace {
@@ -1164,6 +1172,33 @@ __global__ __launch_bounds__(256) void spmm_tcgnn_kernel(
 
 using namespace dtc;
 
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept.
+dtc_status_t noalloc_scratch(void **out) {
+  int dev = 0;
+  DTC_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(DTC_ERROR_UNSUPPORTED, "device ordinal " + std::to_string(dev) + " beyond 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(DTC_ERROR_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return DTC_SUCCESS;
+}
+
+}  // namespace
+
 extern "C" {
 
 dtc_status_t dtc_spmm_tcgnn(const int32_t *node_pointer, const int32_t *edge_list,
@@ -1375,6 +1410,38 @@ dtc_status_t dtc_spmm_balanced(const int32_t *tcblock_rowid, const uint8_t *tcbl
     DTC_LAUNCH_CHECK("balance_fixup");
   }
   return DTC_SUCCESS;
+}
+
+// dtc_spmm_balanced with the library's fixed scratch array as its workspace:
+// no hipMalloc/hipFree and no allocating call.  Everything the kernels read
+// from the scratch is written earlier in the same call (chunk runs and the
+// persistent counter by the preparation kernel, partial tiles by the SpMM
+// kernel), so nothing needs clearing between calls.
+dtc_status_t dtc_spmm_balanced_noalloc(const int32_t *tcblock_rowid, const uint8_t *tcblocktile_id,
+                                       const int32_t *tcblock_offset, const int32_t *sparse_atox,
+                                       const float *tcblock_values, int32_t num_blocks, int32_t nnz,
+                                       int32_t num_nodes, const void *b, int32_t b_rows, int32_t n,
+                                       float *c, const dtc_spmm_options_t *options,
+                                       dtc_stream_t stream) {
+  clear_error();
+  if (num_blocks < 0 || n < 0) return fail(DTC_ERROR_INVALID_VALUE, "num_blocks and N must be >= 0");
+  if (options && options->chunk_blocks < 0)
+    return fail(DTC_ERROR_INVALID_VALUE, "chunk_blocks must be >= 0");
+  const bool det = options && options->deterministic;
+  const int chunk = (options && options->chunk_blocks > 0) ? options->chunk_blocks
+                                                           : default_chunk_blocks(num_blocks);
+  const size_t need = balanced_workspace_bytes_for(num_blocks, n, chunk, det);
+  if (need > kNoAllocScratchBytes)
+    return fail(DTC_ERROR_UNSUPPORTED,
+                "this SpMM needs " + std::to_string(need) + " bytes of workspace, more than the " +
+                    std::to_string(kNoAllocScratchBytes) +
+                    " bytes of the allocation-free entry point; use dtc_spmm_balanced with a workspace");
+  void *ws = nullptr;
+  dtc_status_t st = noalloc_scratch(&ws);
+  if (st != DTC_SUCCESS) return st;
+  return dtc_spmm_balanced(tcblock_rowid, tcblocktile_id, tcblock_offset, sparse_atox, tcblock_values,
+                           num_blocks, nnz, num_nodes, b, b_rows, n, c, options, ws, kNoAllocScratchBytes,
+                           stream);
 }
 
 }  // extern "C"

@@ -51,6 +51,8 @@ _SIGS = {
     "dtc_spmm_balanced_workspace_size_ex": ([i32, i32, ctypes.POINTER(SpmmOptions), ctypes.POINTER(sz)], i32),
     "dtc_spmm_balanced": ([vp, vp, vp, vp, vp, i32, i32, i32, vp, i32, i32, vp,
                            ctypes.POINTER(SpmmOptions), vp, sz, vp], i32),
+    "dtc_spmm_balanced_noalloc": ([vp, vp, vp, vp, vp, i32, i32, i32, vp, i32, i32, vp,
+                                   ctypes.POINTER(SpmmOptions), vp], i32),
 }
 
 
@@ -126,3 +128,10 @@ class Library:
         self.call("dtc_spmm_balanced", tcblock_rowid, tcblocktile_id, tcblock_offset, sparse_atox,
                   tcblock_values, num_blocks, nnz, num_nodes, b, b_rows, n, c, ctypes.byref(opt), workspace,
                   workspace_bytes, stream)
+
+    def spmm_balanced_noalloc(self, tcblock_rowid, tcblocktile_id, tcblock_offset, sparse_atox, tcblock_values,
+                              num_blocks, nnz, num_nodes, b, b_rows, n, c, options=None, stream=0) -> None:
+        """The balanced SpMM on the library's fixed 8 MiB scratch (no allocation)."""
+        opt = options or self.options()
+        self.call("dtc_spmm_balanced_noalloc", tcblock_rowid, tcblocktile_id, tcblock_offset, sparse_atox,
+                  tcblock_values, num_blocks, nnz, num_nodes, b, b_rows, n, c, ctypes.byref(opt), stream)

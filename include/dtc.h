@@ -192,6 +192,28 @@ dtc_status_t dtc_spmm_balanced(const int32_t *tcblock_rowid, const uint8_t *tcbl
                                size_t workspace_bytes, dtc_stream_t stream);
 
 /*
+ * dtc_spmm_balanced without a workspace argument, allocating nothing (no
+ * hipMalloc, hipFree or allocating library call).  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory -- code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  The workspace is a fixed 8 MiB device array inside the library,
+ * one copy per GPU, whose address is looked up on the first call on each
+ * device (the stream must belong to the current device) and which is held
+ * until the process ends.  A call whose workspace
+ * (dtc_spmm_balanced_workspace_size_ex) exceeds 8 MiB -- about a million
+ * chunks, or a deterministic merge whose partial tiles do not fit -- returns
+ * DTC_ERROR_UNSUPPORTED before any work.  Every call on one GPU uses the same
+ * array: calls that overlap in time on one GPU (on different streams, or from
+ * different threads without ordering) race and must not be issued.
+ */
+dtc_status_t dtc_spmm_balanced_noalloc(const int32_t *tcblock_rowid, const uint8_t *tcblocktile_id,
+                                       const int32_t *tcblock_offset, const int32_t *sparse_atox,
+                                       const float *tcblock_values, int32_t num_blocks, int32_t nnz,
+                                       int32_t num_nodes, const void *b, int32_t b_rows, int32_t n,
+                                       float *c, const dtc_spmm_options_t *options,
+                                       dtc_stream_t stream);
+
+/*
  * TC-GNN's SpMM, the comparison baseline without TC-block compaction
  * (replaces run_TCGNNSpMM / spmm_forward_cuda, DTCSpMM.cpp:97-115,
  * DTCSpMM_kernel.cu:703-808).  Its format is the phase-1 output of the ME-TCF
