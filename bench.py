@@ -292,7 +292,7 @@ def main():
     from dtc import selector as sel
     from dtc.graphs import rmat, sbm_hubs, uniform_rows
     from dtc.matrix import DTCMatrix
-    from dtc.metrics import compulsory_bytes, gather_model_bytes, spmm_flops
+    from dtc.metrics import compulsory_bytes, gather_model_bytes, spmm_flops, spmm_source_sha16
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
@@ -359,6 +359,11 @@ def main():
 
     def runner(v):
         sched, prec, chunk, vec, sub, pers = v
+        if sched == "tcgnn":  # TC-GNN's SpMM (no TC-block compaction), float32
+            from dtc import load_extension
+            ext = load_extension()
+            return lambda: C.copy_(ext.run_TCGNNSpMM(B, A.rowptr, A.col, A.block_partition, A.edge_to_column,
+                                                     A.edge_to_row, values=A.values)[0][:M])
         return lambda: A.spmm(B, schedule=sched, precision=prec, chunk_blocks=chunk, vec=vec,
                               subtiles=sub, persistent=pers, out=C)
 
@@ -395,6 +400,20 @@ def main():
                               gflops=spmm_flops(nnz, N) / ms / 1e6, err=err_cache[v[1]][0],
                               err_norm=err_cache[v[1]][1]))
             log(f"  {sel.variant_name(v):42s} {ms:8.3f} ms  err={err_cache[v[1]][0]:.2e}")
+        # TC-GNN's SpMM as one more row of the exhaustive search (the baseline
+        # without TC-block compaction; seconds per call on power-law graphs,
+        # minutes on the block model, where it is left out)
+        if args.workload != "sbm":
+            v = ("tcgnn", "fp32", 0, 0, 0, False)
+            fn = runner(v)
+            fn()
+            torch.cuda.synchronize()
+            e_t = errors(C)
+            ms = timed(stream, fn, 1)
+            table_variants.append(v)
+            table.append(dict(variant="tcgnn:fp32", precision="fp32", ms=ms,
+                              gflops=spmm_flops(nnz, N) / ms / 1e6, err=e_t[0], err_norm=e_t[1]))
+            log(f"  {'tcgnn:fp32':42s} {ms:8.3f} ms  err={e_t[0]:.2e}")
 
     if args.variant == "selector" and pick is not None and not ood:
         chosen_v = sel.VARIANTS[pick]
@@ -427,11 +446,14 @@ def main():
     gflops = flops / ms / 1e6
     bytes_model = gather_model_bytes(sum_u, M, N, nnz, T, W)
     achieved = bytes_model / (ms * 1e-3) / 1e9
+    # HBM traffic measured by counters (FETCH_SIZE + WRITE_SIZE, calibrated on a
+    # plain copy) for this workload and variant, used only if it was taken on
+    # the current SpMM sources (a stamp of their hash)
     traffic = None
     if args.workload == "rmat" and args.scale == 20 and N == 256 and os.path.exists(args.traffic_file):
         with open(args.traffic_file) as f:
             tj = json.load(f)
-        if tj.get("variant") == sel.variant_name(chosen_v):  # counters of this same kernel
+        if tj.get("variant") == sel.variant_name(chosen_v) and tj.get("source_sha16") == spmm_source_sha16():
             traffic = tj.get("hbm_bytes_per_launch")
 
     # ---- plain FP32 path: fastest FP32 variant, timed like the chosen one
@@ -446,9 +468,26 @@ def main():
         fp32 = {"variant": sel.variant_name(v32), "ms": round(fp32_ms, 4),
                 "gflops": round(flops / fp32_ms / 1e6, 2), "speedup_of_chosen": round(fp32_ms / ms, 3)}
 
+    # ---- mixed-precision path: the fastest non-fp32 variant within the error
+    # gate, timed like the others, and its speed against the fp32 path --
+    # reported whatever the pick is
+    mixed = None
+    mixed_rows = [(r["ms"], i) for i, r in enumerate(table)
+                  if r["precision"] != "fp32" and r["err"] <= ERR_TARGET]
+    if mixed_rows and fp32:
+        i_m = min(mixed_rows)[1]
+        fm = runner(table_variants[i_m])
+        for _ in range(3):
+            fm()
+        m_ms, _ = timed_steps(stream, fm, args.steps)
+        mixed = {"variant": table[i_m]["variant"], "ms": round(m_ms, 4), "gflops": round(flops / m_ms / 1e6, 2),
+                 "err": table[i_m]["err"], "speedup_vs_fp32": round(fp32["ms"] / m_ms, 3), "target_speedup": 1.5,
+                 "fastest_failing_gate": min(((r["ms"], r["variant"], r["err"]) for r in table
+                                              if r["err"] > ERR_TARGET), default=None)}
+
     selector_info = None
     if table:
-        best = min(range(len(sel.VARIANTS)), key=lambda i: table[i]["ms"])
+        best = min(range(len(sel.VARIANTS)), key=lambda i: table[i]["ms"])  # the selector's variants
         selector_info = {"pick": None if pick is None else sel.variant_name(sel.VARIANTS[pick]),
                          "out_of_training_range": ood,
                          "exhaustive_best": sel.variant_name(sel.VARIANTS[best]),
@@ -464,52 +503,41 @@ def main():
                   "SparseAToXindex": A.sparse_atox.cpu().numpy()}
         tcv = A.values_tc.cpu().numpy()
         Cc = np.empty((M, N), np.float32)
-        oracle.spmm_metcf_f32(m_host, Bn, M, tc_values=tcv, nthreads=cores, out=Cc)
-        runs = []
-        for _ in range(3):
-            t = time.perf_counter()
-            oracle.spmm_metcf_f32(m_host, Bn, M, tc_values=tcv, nthreads=cores, out=Cc)
-            runs.append(time.perf_counter() - t)
-        t_cpu = float(np.median(runs))
-        tb = []
-        for _ in range(2):
-            t = time.perf_counter()
-            oracle.metcf(a.rowptr, a.col, M, values=a.values, nthreads=cores)
-            tb.append(time.perf_counter() - t)
-        # torch.sparse CSR product on the same cores, one run after a warm-up
+        reps = 3 if (args.workload == "sbm" and N >= 512) else 5
+
+        def median_time(fn):  # one warm-up, then the median of `reps` runs
+            fn()
+            ts = []
+            for _ in range(reps):
+                t = time.perf_counter()
+                fn()
+                ts.append(time.perf_counter() - t)
+            return float(np.median(ts))
+        t_cpu = median_time(lambda: oracle.spmm_metcf_f32(m_host, Bn, M, tc_values=tcv, nthreads=cores, out=Cc))
+        t_build = median_time(lambda: oracle.metcf(a.rowptr, a.col, M, values=a.values, nthreads=cores))
+        # the host SpMM of the baseline plan: torch.sparse CSR x dense on the same cores
         torch.set_num_threads(cores)
         As = torch.sparse_csr_tensor(torch.from_numpy(a.rowptr.astype(np.int64)),
                                      torch.from_numpy(a.col.astype(np.int64)),
                                      torch.from_numpy(a.values), size=(M, M))
         Bt = torch.from_numpy(Bn)
-        torch.sparse.mm(As, Bt)
-        t = time.perf_counter()
-        torch.sparse.mm(As, Bt)
-        t_ts = time.perf_counter() - t
+        t_ts = median_time(lambda: torch.sparse.mm(As, Bt))
         cpu = {"value": round(flops / t_cpu / 1e9, 2), "unit": "GFLOP/s", "cores": cores, "kind": "port",
                "sample": f"full SpMM of this workload ({args.workload}, N={N}) with the CPU "
-                         f"oracle's float32 ME-TCF port, median of 3 runs; host ME-TCF build "
-                         f"{np.median(tb) * 1e3:.0f} ms", "cpu_model": _cpu_model(),
-               "host_metcf_build_ms": round(float(np.median(tb)) * 1e3, 1),
-               "torch_sparse_csr_gflops": round(flops / t_ts / 1e9, 2)}
+                         f"oracle's float32 ME-TCF port, median of {reps} runs after a warm-up; host "
+                         f"ME-TCF build {t_build * 1e3:.0f} ms and torch.sparse CSR SpMM "
+                         f"{flops / t_ts / 1e9:.1f} GFLOP/s timed the same way", "cpu_model": _cpu_model(),
+               "host_metcf_build_ms": round(t_build * 1e3, 1),
+               "torch_sparse_csr_gflops": round(flops / t_ts / 1e9, 2),
+               "torch_version": torch.__version__, "runs": reps}
 
-    # ---- TC-GNN's SpMM on the same matrix (the comparison baseline without
-    # TC-block compaction; it rescans a window's edges for every group of blocks)
+    # ---- TC-GNN's SpMM on the same matrix (its row of the exhaustive table)
     tcgnn = None
-    if not args.no_table:
-        from dtc import load_extension
-        ext = load_extension()
-
-        def run_tcgnn():
-            return ext.run_TCGNNSpMM(B, A.rowptr, A.col, A.block_partition, A.edge_to_column, A.edge_to_row,
-                                     values=A.values)[0]
-        Ct = run_tcgnn()
-        torch.cuda.synchronize()
-        t_err = errors(Ct)[0]
-        t_ms = timed(stream, run_tcgnn, 1)  # seconds per call on power-law graphs
-        tcgnn = {"ms": round(t_ms, 3), "gflops": round(flops / t_ms / 1e6, 2), "err": t_err,
-                 "chosen_speedup_over_tcgnn": round(t_ms / ms, 2)}
-        log(f"TC-GNN baseline: {t_ms:.3f} ms")
+    t_rows = [r for r in table if r["variant"] == "tcgnn:fp32"]
+    if t_rows:
+        r = t_rows[0]
+        tcgnn = {"ms": round(r["ms"], 3), "gflops": round(r["gflops"], 2), "err": r["err"],
+                 "chosen_speedup_over_tcgnn": round(r["ms"] / ms, 2)}
 
     # ---- practical ceiling: the same B-row loads with nothing else
     ceiling = None
@@ -558,10 +586,14 @@ def main():
                    "sum_unique_columns": sum_u, "variant": sel.variant_name(chosen_v), "chosen_by": how,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_frac": (None if traffic is None
+                                      else round(traffic / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)),
+                     "model": "gather: 4 N sum_w U_w + 4 M N + 5 nnz + 36 T + 4 (W + 1) bytes per launch"},
         "cpu_baseline": cpu,
         "error_vs_fp64": {"componentwise": comp_err, "normwise": norm_err, "target": ERR_TARGET},
         "fp32_path": fp32,
+        "mixed_path": mixed,
         "selector": selector_info,
         "gather_ceiling": ceiling,
         "tcgnn_baseline": tcgnn,

@@ -43,3 +43,20 @@ def error_report(C, C64, Cabs64) -> dict:
     mx = float(np.abs(C64).max()) if C64.size else 0.0
     norm = float(diff.max() / mx) if mx > 0 else float(diff.max() if diff.size else 0.0)
     return {"componentwise": comp, "normwise": norm}
+
+
+SPMM_SOURCES = ("csrc/spmm.hip", "csrc/dtc_internal.h")
+
+
+def spmm_source_sha16() -> str:
+    """First 16 hex digits of the SHA-256 over the SpMM kernel's sources:
+    the stamp that ties a counter measurement (HBM traffic per launch) to the
+    kernels it was taken on."""
+    import hashlib
+    import os
+    pkg = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    h = hashlib.sha256()
+    for rel in SPMM_SOURCES:
+        with open(os.path.join(pkg, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
