@@ -164,3 +164,39 @@ def test_metcf_cache_roundtrip(ext, tmp_path):
         spmm_autograd(vv, Bb, M_).square().sum().backward()
         grads.append((vv.grad, Bb.grad))
     assert torch.equal(grads[0][0], grads[1][0]) and torch.equal(grads[0][1], grads[1][1])
+
+
+@pytest.mark.parametrize("bw", [1, 8])
+def test_mixed_window_sizes(ext, bw):
+    """Windows of very different sizes side by side: one of 65,000 edges over
+    columns up to 3,000,000 (with 1-column blocks, about 60,000 blocks in one
+    window), one of 6,000 edges over 50 columns, one of 3,000 edges, and many
+    small and empty windows."""
+    rng = np.random.default_rng(bw)
+    M = 200
+    rows = [[] for _ in range(M)]
+    rows[3] = list(rng.choice(3_000_000, 60_000, replace=False))          # huge window
+    rows[4] = list(rng.integers(0, 3_000_000, 5_000)) + [7, 7, 7]          # same window, repeats
+    rows[40] = list(rng.integers(0, 50, 6_000))                            # big window, few columns
+    rows[81] = list(rng.integers(0, 4_000, 3_000))                         # a few thousand edges
+    for r in range(100, 200, 3):
+        rows[r] = list(rng.integers(0, 2_999_999, int(rng.integers(0, 40))))
+    rowptr = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int32)
+    col = np.concatenate([np.array(r, np.int64) for r in rows]).astype(np.int32)
+    check(ext, rowptr, col, M, bw=bw, ctx=f"mixed window sizes bw={bw}")
+
+
+@pytest.mark.parametrize("per_window", [4, 8])
+def test_many_short_windows(ext, per_window):
+    """Tens of thousands of windows of a few edges each (random rows of the
+    window, duplicates dropped), so that thousands of consecutive windows
+    share every stretch of 2,048 edges."""
+    rng = np.random.default_rng(per_window)
+    M = 16 * (2048 // per_window) * 12
+    rows = [set() for _ in range(M)]
+    for w in range(M // 16):
+        for _ in range(per_window):
+            rows[16 * w + int(rng.integers(0, 16))].add(int(rng.integers(0, 5000)))
+    rowptr = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int32)
+    col = np.concatenate([np.array(sorted(r), np.int64) for r in rows]).astype(np.int32)
+    check(ext, rowptr, col, M, ctx=f"short windows per_window={per_window}")
