@@ -121,6 +121,40 @@ def build_ext(force: bool = False, verbose: bool = False) -> str:
     return out
 
 
+# Host code of the library under AddressSanitizer and UndefinedBehaviorSanitizer
+# (device code is compiled as usual: -Xarch_host keeps the sanitizers off the
+# gfx950 pass).  The sanitizer runtime is not linked in; the process preloads
+# ROCm clang's ASan runtime, as tests/run_sanitizers.sh does.  CPU use only.
+SAN_DIR = os.path.join(HERE, "_build_san")
+SAN_LIB = os.path.join(SAN_DIR, "libdtc.so")
+SAN_FLAGS = [
+    "-O1", "-g", "-fno-omit-frame-pointer",
+    "-Xarch_host", "-fsanitize=address,undefined",
+    "-Xarch_host", "-fno-sanitize=vptr",
+    "-Xarch_host", "-fno-sanitize-recover=undefined",
+]
+
+
+def build_sanitized(verbose: bool = False, jobs: int = 4) -> str:
+    os.makedirs(SAN_DIR, exist_ok=True)
+    flags = [f for f in HIP_FLAGS if f != "-O3"] + SAN_FLAGS
+    objs, todo = [], []
+    for src in HIP_SOURCES:
+        s = os.path.join(CSRC, src)
+        o = os.path.join(SAN_DIR, src.replace(".hip", ".o"))
+        objs.append(o)
+        if _stale(o, [s] + HEADERS):
+            todo.append([HIPCC, *flags, "-c", s, "-o", o])
+    with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        for f in [ex.submit(_run, c, verbose) for c in todo]:
+            f.result()
+    _, torch_lib, _ = _torch_paths()
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", SAN_LIB + ".tmp",
+          f"-Wl,-rpath,{torch_lib}:{ROCM}/lib", "-Wl,-soname,libdtc.so"], verbose)
+    os.replace(SAN_LIB + ".tmp", SAN_LIB)
+    return SAN_LIB
+
+
 def build(force: bool = False, verbose: bool = False) -> None:
     build_lib(force=force, verbose=verbose)
     build_ext(force=force, verbose=verbose)
@@ -132,7 +166,12 @@ if __name__ == "__main__":
     ap.add_argument("--force", action="store_true")
     ap.add_argument("-v", "--verbose", action="store_true")
     ap.add_argument("--lib-only", action="store_true")
+    ap.add_argument("--sanitize", action="store_true",
+                    help="build only the ASan/UBSan host build of the library (_build_san/libdtc.so)")
     a = ap.parse_args()
+    if a.sanitize:
+        print("built", build_sanitized(verbose=a.verbose))
+        sys.exit(0)
     build_lib(force=a.force, verbose=a.verbose)
     if not a.lib_only:
         build_ext(force=a.force, verbose=a.verbose)

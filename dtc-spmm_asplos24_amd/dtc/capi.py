@@ -16,7 +16,10 @@ from __future__ import annotations
 import ctypes
 import os
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "libdtc.so")
+# DTC_LIBRARY selects another build of the library (e.g. the sanitized host
+# build that build.py --sanitize writes under _build_san/).
+LIB_PATH = os.environ.get("DTC_LIBRARY") or os.path.join(
+    os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "libdtc.so")
 
 i32, i64, vp, sz = ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p, ctypes.c_size_t
 

@@ -16,7 +16,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "libmetcf_oracle.so")
+# DTC_ORACLE_LIBRARY selects another build of the same source (the sanitized
+# one under _build/san/, see the Makefile's `sanitize` target).
+_LIB_PATH = os.environ.get("DTC_ORACLE_LIBRARY") or os.path.join(_HERE, "_build", "libmetcf_oracle.so")
 _lib = None
 
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")

@@ -11,7 +11,8 @@ import torch
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(REPO, "dtc-spmm_asplos24_amd")
-LIB = os.path.join(PKG, "libdtc.so")
+# DTC_LIBRARY: another build of the library (the sanitized host build)
+LIB = os.environ.get("DTC_LIBRARY") or os.path.join(PKG, "libdtc.so")
 HEADER = os.path.join(REPO, "include", "dtc.h")
 
 
