@@ -222,7 +222,7 @@ def bench_reordered(a, B, C64, Cabs, mask, v, ms, flops, sum_u, W, N, stream, st
 
     def fr():
         return Ar.spmm(Br, schedule=v[0], precision=v[1], chunk_blocks=v[2], vec=v[3], subtiles=v[4],
-                       persistent=v[5], out=Cr)
+                       persistent=v[5], out=Cr, cut_span_blocks=v[6])
     for _ in range(3):
         fr()
     r_ms = timed(stream, fr, steps)
@@ -258,7 +258,8 @@ def main():
     ap.add_argument("--n", type=int, default=0,
                     help="columns of B (default: 256 for rmat and gcn, 128 for uniform and sbm)")
     ap.add_argument("--variant", default="selector",
-                    help="selector (learned pick), exhaustive, or schedule:precision:chunk:vec:subtiles")
+                    help="selector (learned pick), exhaustive, or "
+                         "schedule:precision:chunk:vec:subtiles[:cut_span]")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-table", action="store_true", help="skip the exhaustive variant table")
     ap.add_argument("--traffic-file", default=os.path.join(REPO, "profiles", "traffic_rmat20_n256.json"))
@@ -358,14 +359,14 @@ def main():
         return comp, norm
 
     def runner(v):
-        sched, prec, chunk, vec, sub, pers = v
+        sched, prec, chunk, vec, sub, pers, span = v
         if sched == "tcgnn":  # TC-GNN's SpMM (no TC-block compaction), float32
             from dtc import load_extension
             ext = load_extension()
             return lambda: C.copy_(ext.run_TCGNNSpMM(B, A.rowptr, A.col, A.block_partition, A.edge_to_column,
                                                      A.edge_to_row, values=A.values)[0][:M])
         return lambda: A.spmm(B, schedule=sched, precision=prec, chunk_blocks=chunk, vec=vec,
-                              subtiles=sub, persistent=pers, out=C)
+                              subtiles=sub, persistent=pers, out=C, cut_span_blocks=span)
 
     # ---- the learned selector's pick
     feats = sel.features(A, N)
@@ -381,11 +382,11 @@ def main():
     # ---- exhaustive table (the selector's yardstick and fallback)
     table_variants = list(sel.VARIANTS)
     if not args.no_table:
-        table_variants += [("balanced", p, 256, 4, 1, False) for p in EXTRA_PRECISIONS]
-        table_variants += [("balanced", "fp32", c, 2, 1, False) for c in EXTRA_SMALL_CHUNKS]
+        table_variants += [("balanced", p, 256, 4, 1, False, 0) for p in EXTRA_PRECISIONS]
+        table_variants += [("balanced", "fp32", c, 2, 1, False, 0) for c in EXTRA_SMALL_CHUNKS]
         for c in filter(None, args.chunks.split(",")):
             chunk, _, vec = c.partition(":")
-            table_variants.append(("balanced", "fp32", int(chunk), int(vec or 4), 1, False))
+            table_variants.append(("balanced", "fp32", int(chunk), int(vec or 4), 1, False, 0))
     table = []
     err_cache = {}
     if not args.no_table:
@@ -404,7 +405,7 @@ def main():
         # without TC-block compaction; seconds per call on power-law graphs,
         # minutes on the block model, where it is left out)
         if args.workload != "sbm":
-            v = ("tcgnn", "fp32", 0, 0, 0, False)
+            v = ("tcgnn", "fp32", 0, 0, 0, False, 0)
             fn = runner(v)
             fn()
             torch.cuda.synchronize()
@@ -429,8 +430,8 @@ def main():
         how = "exhaustive" + (f" (selector fallback: {', '.join(ood)} outside its training range)"
                               if ood and args.variant == "selector" else "")
     else:
-        s_, p_, c_, vec_, sub_ = args.variant.split(":")
-        chosen_v = (s_, p_, int(c_), int(vec_), int(sub_), False)
+        s_, p_, c_, vec_, sub_, *span_ = args.variant.split(":")
+        chosen_v = (s_, p_, int(c_), int(vec_), int(sub_), False, int(span_[0]) if span_ else 0)
         how = "fixed"
 
     # ---- timed region: K steps of the chosen variant

@@ -405,12 +405,14 @@ std::vector<Tensor> run_DTCSpMM(Tensor input, Tensor Rowwindow_offset, Tensor TC
 std::vector<Tensor> run_DTCSpMM_balance(Tensor input, Tensor TCblock_rowid, Tensor TCblocktile_id,
                                         Tensor TCblock_offset, Tensor sparse_AToX_idx, int64_t num_nodes,
                                         std::string exeplan, c10::optional<Tensor> values, std::string precision,
-                                        int64_t chunk_blocks, bool persistent, bool deterministic) {
+                                        int64_t chunk_blocks, bool persistent, bool deterministic,
+                                        int64_t cut_span_blocks) {
   CHECK_DTYPE(input, torch::kFloat32);
   dtc_spmm_options_t opt{};
   parse_exeplan(exeplan, opt);
   opt.precision = parse_precision(precision);
   opt.chunk_blocks = (int32_t)chunk_blocks;
+  opt.cut_span_blocks = (int32_t)cut_span_blocks;
   opt.persistent = persistent ? 1 : 0;
   opt.deterministic = deterministic ? 1 : 0;
   Operands o{input, TCblocktile_id, TCblock_offset, sparse_AToX_idx, values};
@@ -439,9 +441,10 @@ std::vector<Tensor> DTCSpMM_gcn(Tensor input, Tensor Rowwindow_offset, Tensor TC
 Tensor spmm(Tensor input, Tensor window_or_rowid, Tensor TCblocktile_id, Tensor TCblock_offset, Tensor sparse_AToX_idx,
             int64_t num_nodes, std::string schedule, c10::optional<Tensor> values, std::string precision, int64_t vec,
             int64_t subtiles, int64_t chunk_blocks, bool persistent, c10::optional<Tensor> out,
-            bool deterministic) {
+            bool deterministic, int64_t cut_span_blocks) {
   dtc_spmm_options_t opt{};
   opt.deterministic = deterministic ? 1 : 0;
+  opt.cut_span_blocks = (int32_t)cut_span_blocks;
   opt.precision = parse_precision(precision);
   TORCH_CHECK_VALUE(vec == 0 || vec == 1 || vec == 2 || vec == 4 || vec == 8, "vec must be 0, 1, 2, 4 or 8");
   TORCH_CHECK_VALUE(subtiles >= 0 && subtiles <= 2, "subtiles must be 0, 1 or 2");
@@ -777,7 +780,8 @@ PYBIND11_MODULE(DTCSpMM, m) {
   m.def("run_DTCSpMM_balance", &run_DTCSpMM_balance, "DTC-SpMM forward, balanced TC-block chunks", py::arg("input"),
         py::arg("TCblock_rowid"), py::arg("TCblocktile_id"), py::arg("TCblock_offset"), py::arg("sparse_AToX_idx"),
         py::arg("num_nodes"), py::arg("exeplan"), py::arg("values") = py::none(), py::arg("precision") = "fp32",
-        py::arg("chunk_blocks") = 0, py::arg("persistent") = false, py::arg("deterministic") = false);
+        py::arg("chunk_blocks") = 0, py::arg("persistent") = false, py::arg("deterministic") = false,
+        py::arg("cut_span_blocks") = 0);
   m.def("DTCSpMM_gcn", &DTCSpMM_gcn, "DTC-SpMM forward, single launch, variant chosen from N", py::arg("input"),
         py::arg("Rowwindow_offset"), py::arg("TCblocktile_id"), py::arg("TCblock_offset"), py::arg("sparse_AToX_idx"),
         py::arg("num_nodes"), py::arg("num_edges"), py::arg("values") = py::none());
@@ -796,12 +800,12 @@ PYBIND11_MODULE(DTCSpMM, m) {
         py::arg("input"), py::arg("TCblock_rowid"), py::arg("TCblocktile_id"), py::arg("TCblock_offset"),
         py::arg("sparse_AToX_idx"), py::arg("num_nodes"), py::arg("exeplan"), py::arg("values") = py::none(),
         py::arg("precision") = "fp32", py::arg("chunk_blocks") = 0, py::arg("persistent") = false,
-        py::arg("deterministic") = false);
+        py::arg("deterministic") = false, py::arg("cut_span_blocks") = 0);
   m.def("spmm", &spmm, "SpMM with explicit schedule and tuning knobs", py::arg("input"), py::arg("window_or_rowid"),
         py::arg("TCblocktile_id"), py::arg("TCblock_offset"), py::arg("sparse_AToX_idx"), py::arg("num_nodes"),
         py::arg("schedule") = "balanced", py::arg("values") = py::none(), py::arg("precision") = "fp32",
         py::arg("vec") = 0, py::arg("subtiles") = 0, py::arg("chunk_blocks") = 0, py::arg("persistent") = false,
-        py::arg("out") = py::none(), py::arg("deterministic") = false);
+        py::arg("out") = py::none(), py::arg("deterministic") = false, py::arg("cut_span_blocks") = 0);
   m.def("csr_transpose", &csr_transpose, "CSR transpose on the GPU -> (rowptr_t, col_t, perm)",
         py::arg("nodePointer"), py::arg("edgeList"), py::arg("num_rows"), py::arg("num_cols"));
   m.def("sddmm_csr", &sddmm_csr, "out[e] = X[row(e)] . Y[col(e)]", py::arg("nodePointer"), py::arg("edgeList"),

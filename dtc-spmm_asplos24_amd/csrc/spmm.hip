@@ -86,6 +86,7 @@ struct SpmmParams {
   int32_t *queue;      // persistent work counter
   const int2 *runs;    // balanced: run [b, e) of every chunk, cut flags in the sign bits
   float *parts;        // balanced, deterministic merge: partial-tile slots (NULL: atomics)
+  int32_t span;        // balanced cut span in TC blocks (a multiple of chunk)
 };
 
 // Kernels take the arrays as restrict-qualified arguments: the compiler may
@@ -100,7 +101,7 @@ struct SpmmParams {
   SpmmParams {                                                                             \
     a_rwo, a_rowid, a_tile, a_off, a_atox, a_vals, a_B, a_C, a_p.num_windows,              \
         a_p.num_blocks, a_p.M, a_p.b_rows, a_p.N, a_p.nnz, a_p.n_ct, a_p.num_items,        \
-        a_p.chunk, a_p.queue, a_p.runs, a_p.parts                                          \
+        a_p.chunk, a_p.queue, a_p.runs, a_p.parts, a_p.span                                \
   }
 
 template <int VEC>
@@ -812,25 +813,34 @@ __device__ __forceinline__ int window_end_near(const int32_t *rowid, int c, int 
   return -1;
 }
 
-// Balanced-schedule preparation.  Ownership rule: a window of at most `chunk`
-// blocks belongs whole to the chunk holding its first block (and may run
-// past that chunk's end); longer windows are cut at chunk boundaries and
-// their pieces merged with atomic adds.  Wave v, one per chunk:
+// Balanced-schedule preparation.  Ownership rule, for chunks of `chunk` TC
+// blocks and a cut span `span` (a multiple of chunk): a window of at most
+// `span` blocks belongs whole to the chunk holding its first block (and may
+// run past that chunk's end); a longer window is cut at the multiples of
+// `span` into pieces, each belonging to the chunk holding its first block,
+// and the pieces are merged with atomic adds.  The multiples of `span` are
+// chunk boundaries, so a chunk's run can only begin with a cut piece (one that
+// starts at the chunk's first block) and only end with one (the first piece
+// of a long window that starts inside the chunk, which then runs past the
+// chunk's end); every window in between is short and whole.  A larger span
+// cuts fewer windows into fewer pieces (fewer atomic tiles written) at the
+// price of longer work units.  Wave v, one per chunk:
 //   * for chunk v ([c0, c1) = blocks [v*chunk, (v+1)*chunk)), writes the
-//     chunk's run [b, e) to runs[v]: b = c0 unless c0 lies inside a short
-//     window that began earlier (then b = that window's end), e = c1 unless
-//     a short window that began inside the chunk runs past c1 (then e = that
-//     window's end); ~b marks a cut first window, ~e a cut last window (b >= e:
-//     nothing to do); and zeroes the rows of the cut windows that start in the
-//     chunk, since no chunk stores them whole;
+//     chunk's run [b, e) to runs[v]: b = c0 unless the window or piece
+//     holding c0 began earlier (then b = its end), e = c1 unless a window or
+//     piece that begins inside the chunk runs past c1 (then e = its end); ~b
+//     marks a cut first piece, ~e a cut last piece (b >= e: nothing to do);
+//     and zeroes the rows of the cut windows that start in the chunk, since
+//     no chunk stores them whole;
 //   * (wave 0) resets the persistent work counter.
 // The window searches run here, once per chunk, on an idle memory system
 // instead of at the start of every SpMM wave.  Empty windows are zeroed by
 // the SpMM waves themselves (walk_blocks): nothing else writes them.
 __global__ __launch_bounds__(256) void k_balance_prep(const int32_t *rowid, int32_t T,
                                                       int32_t M, int32_t N,
-                                                      int32_t chunk, int32_t num_items,
-                                                      float *C, int32_t *queue, int2 *runs,
+                                                      int32_t chunk, int32_t span,
+                                                      int32_t num_items, float *C,
+                                                      int32_t *queue, int2 *runs,
                                                       int32_t zero_cut) {
   if (queue && blockIdx.x == 0 && threadIdx.x == 0) *queue = 0;
   const int lane = lane_id();
@@ -838,50 +848,83 @@ __global__ __launch_bounds__(256) void k_balance_prep(const int32_t *rowid, int3
   if (v >= num_items) return;
   const int c0 = (int)v * chunk;
   const int c1 = min(c0 + chunk, T);
-  // Window extents: exact for chunks above kNearSpan (searches); for smaller
-  // chunks only within `chunk` blocks of the boundary, -1 meaning farther --
-  // which already makes the window longer than a chunk.
-  const bool near = chunk <= kNearSpan;
+  // Window extents: exact for spans above kNearSpan (searches); for smaller
+  // spans only within `span` blocks of the boundary, -1 meaning farther --
+  // which already makes the window longer than the span.
+  const bool near = span <= kNearSpan;
   auto start_of = [&](int c, int w) {
-    return wave_uniform(near ? window_start_near(rowid, c, w, chunk) : window_start_from(rowid, c, w));
+    return wave_uniform(near ? window_start_near(rowid, c, w, span) : window_start_from(rowid, c, w));
   };
   auto end_of = [&](int c, int w) {
-    return wave_uniform(near ? window_end_near(rowid, c, T, w, chunk)
+    return wave_uniform(near ? window_end_near(rowid, c, T, w, span)
                              : max(c + 1, window_end_from(rowid, c, T, w)));
   };
+  // first multiple of span after x (a chunk boundary; no multiple of span
+  // lies strictly inside a chunk)
+  auto next_cut = [&](int x) { return (int)min<int64_t>(((int64_t)x / span + 1) * span, T); };
+  auto is_long = [&](int ws, int we) { return ws < 0 || we < 0 || we - ws > span; };
   const int w0 = wave_uniform(rowid[c0]);
   const int ws0 = start_of(c0, w0);
   const int we0 = end_of(c0, w0);
-  const bool first_long = ws0 < 0 || we0 < 0 || we0 - ws0 > chunk;
-  int b = (!first_long && ws0 < c0) ? we0 : c0;
+  const bool first_long = is_long(ws0, we0);
+  bool cut_first = false;
+  int b;
+  if (!first_long) {
+    b = ws0 < c0 ? we0 : c0;
+  } else {
+    // the piece holding c0 starts at c0 when c0 is a multiple of span or the
+    // window's first block; it ends at the next multiple of span or the
+    // window's end (an unknown end lies more than span away: past the cut)
+    const int pe = (we0 >= 0 && we0 < next_cut(c0)) ? we0 : next_cut(c0);
+    if (c0 % span == 0 || ws0 == c0) {
+      b = c0;
+      cut_first = true;
+    } else {
+      b = pe;
+    }
+  }
   int e = c1;
-  bool last_long = false;
+  bool cut_last = false;
   int wl = -1, wsl = -1;
-  if (c1 < T) {
+  if (b < c1 && c1 < T) {
     const int r_in = wave_uniform(rowid[c1 - 1]);
     const int r_out = wave_uniform(rowid[c1]);
     if (r_in == r_out) {  // a window crosses c1
-      wl = r_in;
-      wsl = r_in == w0 ? ws0 : start_of(c1, r_in);
-      const int wel = r_in == w0 ? we0 : end_of(c1, r_in);
-      if (wsl < 0 || wel < 0 || wel - wsl > chunk) last_long = true;
-      else e = wel;
+      if (r_in == w0) {
+        // the run's first window or piece runs past c1 (b == c0 here): a
+        // short window whole, or the same piece as the one at c0
+        e = first_long ? ((we0 >= 0 && we0 < next_cut(c0)) ? we0 : next_cut(c0)) : we0;
+      } else {
+        wl = r_in;  // starts inside the chunk: within span of c1
+        wsl = start_of(c1, r_in);
+        const int wel = end_of(c1, r_in);
+        if (!is_long(wsl, wel)) {
+          e = wel;
+        } else {
+          e = (wel >= 0 && wel < next_cut(wsl)) ? wel : next_cut(wsl);
+          cut_last = true;
+        }
+      }
     }
   }
-  if (b >= c1) b = e = c1;
-  if (lane == 0) runs[v] = make_int2(first_long ? ~b : b, last_long ? ~e : e);
+  if (b >= c1) {
+    b = e = c1;
+    cut_first = cut_last = false;
+  }
+  if (lane == 0) runs[v] = make_int2(cut_first ? ~b : b, cut_last ? ~e : e);
   if (!zero_cut) return;  // deterministic merge: k_balance_fixup writes cut windows whole
-  if (first_long && ws0 == c0) zero_window_rows(C, w0, M, N);
-  if (last_long && wsl >= c0 && !(wl == w0 && ws0 == c0)) zero_window_rows(C, wl, M, N);
+  if (cut_first && ws0 == c0) zero_window_rows(C, w0, M, N);
+  if (cut_last) zero_window_rows(C, wl, M, N);
 }
 
 // Deterministic merge of cut windows, after the SpMM kernel: wave (chunk c,
 // column tile ct) takes the one cut window that starts in chunk c, if any
-// (a cut window is longer than a chunk, so no chunk holds the start of two),
+// (a cut window is longer than the span, so no chunk holds the start of two),
 // sums its partial tiles in chunk order -- chunk c's slot for the window's
-// role there, then slot 2 c' of every later chunk c' it reaches, where it is
-// the first window -- and stores the window's rows.  Tiles are row-major
-// 16 x tw floats (tw = the SpMM's column-tile width).
+// role there, then slot 2 c' of every chunk c' holding a later piece (the
+// chunks at the multiples of span inside the window), where it is the first
+// piece -- and stores the window's rows.  Tiles are row-major 16 x tw floats
+// (tw = the SpMM's column-tile width).
 __global__ __launch_bounds__(256) void k_balance_fixup(const int32_t *__restrict__ rowid,
                                                        SpmmParams p, int32_t tw) {
   const int wave = wave_uniform(threadIdx.x >> 6);
@@ -910,6 +953,11 @@ __global__ __launch_bounds__(256) void k_balance_fixup(const int32_t *__restrict
   }
   if (w < 0) return;
   const int we = wave_uniform(window_end_from(rowid, e - 1, p.num_blocks, w));
+  // later pieces start at the multiples of span in (c0, we): chunks
+  // first_later, first_later + step, ... (no multiple of span lies strictly
+  // inside chunk c, so the first one after c0 is the first after the start)
+  const int step = p.span / p.chunk;
+  const int first_later = (int)(((int64_t)c0 / p.span + 1) * p.span / p.chunk);
   const int c_last = (we - 1) / p.chunk;
   const int64_t tile = (int64_t)kBlkH * tw;
   const int64_t slot_stride = (int64_t)p.n_ct * tile;
@@ -919,7 +967,7 @@ __global__ __launch_bounds__(256) void k_balance_fixup(const int32_t *__restrict
   for (int k = lane; k < kBlkH * tw; k += kWave) {
     float sum = first[k];
 #pragma unroll 8
-    for (int cc = c + 1; cc <= c_last; ++cc) sum += later[(int64_t)2 * cc * slot_stride + k];
+    for (int cc = first_later; cc <= c_last; cc += step) sum += later[(int64_t)2 * cc * slot_stride + k];
     const int64_t row = (int64_t)w * kBlkH + k / tw;
     const int col = ct * tw + k % tw;
     if (row < p.M && col < p.N) __builtin_nontemporal_store(sum, p.C + row * N + col);
@@ -1056,6 +1104,18 @@ int default_chunk_blocks(int32_t num_blocks) {
   int c = 32;
   while (c < 512 && (int64_t)c * 2 * 8192 <= (int64_t)num_blocks) c *= 2;
   return c;
+}
+
+// Cut span of the balanced schedule: options->cut_span_blocks, a positive
+// multiple of the chunk, or the chunk itself when 0.
+dtc_status_t resolve_cut_span(const dtc_spmm_options_t *opt, int chunk, int *span) {
+  const int s = opt ? opt->cut_span_blocks : 0;
+  if (s < 0) return fail(DTC_ERROR_INVALID_VALUE, "cut_span_blocks must be >= 0");
+  if (s > 0 && s % chunk != 0)
+    return fail(DTC_ERROR_INVALID_VALUE, "cut_span_blocks must be a multiple of the chunk (" +
+                                             std::to_string(chunk) + " blocks)");
+  *span = s > 0 ? s : chunk;
+  return DTC_SUCCESS;
 }
 
 // Balanced workspace: the persistent work counter, then one int2 run per
@@ -1347,6 +1407,9 @@ dtc_status_t dtc_spmm_balanced(const int32_t *tcblock_rowid, const uint8_t *tcbl
   if (st != DTC_SUCCESS) return st;
   const int chunk = (options && options->chunk_blocks > 0) ? options->chunk_blocks
                                                            : default_chunk_blocks(num_blocks);
+  int span = chunk;
+  st = resolve_cut_span(options, chunk, &span);
+  if (st != DTC_SUCCESS) return st;
   const bool persistent = options && options->persistent;
   if (options && (options->deterministic < 0 || options->deterministic > 1))
     return fail(DTC_ERROR_INVALID_VALUE, "deterministic must be 0 or 1");
@@ -1376,6 +1439,7 @@ dtc_status_t dtc_spmm_balanced(const int32_t *tcblock_rowid, const uint8_t *tcbl
   p.n_ct = (int32_t)ceil_div(n, 16 * var.vec * var.nsub);
   p.num_items = (int32_t)ceil_div(num_blocks, chunk);
   p.chunk = chunk;
+  p.span = span;
   p.queue = persistent ? static_cast<int32_t *>(workspace) : nullptr;
   int2 *runs = reinterpret_cast<int2 *>(static_cast<char *>(workspace) + kQueueBytes);
   p.runs = runs;
@@ -1384,7 +1448,7 @@ dtc_status_t dtc_spmm_balanced(const int32_t *tcblock_rowid, const uint8_t *tcbl
                 : nullptr;
   const int64_t prep_waves = p.num_items;
   k_balance_prep<<<(unsigned)ceil_div(prep_waves, kWavesPerBlock), 256, 0, stream>>>(
-      tcblock_rowid, num_blocks, num_nodes, n, chunk, p.num_items, c, p.queue, runs, det ? 0 : 1);
+      tcblock_rowid, num_blocks, num_nodes, n, chunk, span, p.num_items, c, p.queue, runs, det ? 0 : 1);
   DTC_LAUNCH_CHECK("balance_prep");
   if (!persistent) {
     KernelFn fn = pick_kernel(var, 1);

@@ -29,7 +29,8 @@ PRECISIONS = {"fp32": 0, "bf16": 1, "fp16": 2, "bf16x3": 3, "bf16x6": 4}
 class SpmmOptions(ctypes.Structure):
     """dtc_spmm_options_t"""
     _fields_ = [("precision", i32), ("vec", i32), ("subtiles", i32), ("chunk_blocks", i32),
-                ("persistent", i32), ("b_dtype", i32), ("deterministic", i32)]
+                ("persistent", i32), ("b_dtype", i32), ("deterministic", i32),
+                ("cut_span_blocks", i32)]
 
 
 class DTCError(RuntimeError):
@@ -104,9 +105,9 @@ class Library:
     # ----------------------------------------------------------------- SpMM
     @staticmethod
     def options(precision="fp32", vec=0, subtiles=0, chunk_blocks=0, persistent=False, b_dtype=0,
-                deterministic=False) -> SpmmOptions:
+                deterministic=False, cut_span_blocks=0) -> SpmmOptions:
         return SpmmOptions(PRECISIONS[precision], vec, subtiles, chunk_blocks, int(persistent), b_dtype,
-                           int(deterministic))
+                           int(deterministic), cut_span_blocks)
 
     def spmm_windows(self, row_window_offset, tcblocktile_id, tcblock_offset, sparse_atox, tcblock_values,
                      num_windows, num_blocks, nnz, num_nodes, b, b_rows, n, c, options=None, stream=0) -> None:

@@ -83,13 +83,17 @@ class DTCMatrix:
     def spmm(self, B: torch.Tensor, schedule: str = "auto", precision: str = "fp32",
              values_tc: torch.Tensor | None = None, use_values: bool = True, vec: int = 0,
              subtiles: int = 0, chunk_blocks: int = 0, persistent: bool = False,
-             out: torch.Tensor | None = None, deterministic: bool = False) -> torch.Tensor:
+             out: torch.Tensor | None = None, deterministic: bool = False,
+             cut_span_blocks: int = 0) -> torch.Tensor:
         """C = A * B.  schedule "auto" runs the unbalanced schedule when no row
         window is longer than ``dtc.selector.AUTO_WINDOW_BLOCKS`` TC blocks
         and the balanced one otherwise; "windows" / "balanced" force one.
         ``deterministic`` (balanced schedule) merges windows cut across chunks
         in a fixed order, so repeated calls are bitwise identical; the
-        unbalanced schedule is deterministic as it is."""
+        unbalanced schedule is deterministic as it is.  ``cut_span_blocks``
+        (balanced schedule, a multiple of the chunk; 0 = the chunk): windows
+        of at most this many TC blocks are never cut, longer ones are cut at
+        its multiples."""
         ext = load_extension()
         if schedule == "auto":
             from .selector import default_schedule
@@ -100,7 +104,7 @@ class DTCMatrix:
                         self.sparse_atox, self.num_rows, schedule=schedule, values=vals,
                         precision=precision, vec=vec, subtiles=subtiles,
                         chunk_blocks=chunk_blocks, persistent=persistent, out=out,
-                        deterministic=deterministic)
+                        deterministic=deterministic, cut_span_blocks=cut_span_blocks)
 
     # ------------------------------------------------------------- transpose
     def transpose(self) -> "DTCMatrix":

@@ -30,30 +30,37 @@ DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 DEFAULT_DATA = os.path.join(DATA_DIR, "selector_timings.npz")
 DEFAULT_MODEL = os.path.join(DATA_DIR, "selector_mlp.npz")
 
-# (schedule, precision, chunk_blocks, vec, subtiles, persistent).  Every
-# precision here meets a 1e-5 componentwise error bound against float64.
+# (schedule, precision, chunk_blocks, vec, subtiles, persistent,
+# cut_span_blocks; 0 = the chunk).  Every precision here meets a 1e-5
+# componentwise error bound against float64.  The cut-span variants keep
+# windows of up to 256 (512) blocks whole: fewer windows merged with atomic
+# adds, at the price of longer work units.
 VARIANTS = [
-    ("windows", "fp32", 0, 4, 1, False),
-    ("windows", "fp32", 0, 4, 2, False),
-    ("windows", "fp32", 0, 2, 1, False),
-    ("windows", "bf16x6", 0, 4, 1, False),
-    ("balanced", "fp32", 32, 2, 1, False),
-    ("balanced", "fp32", 64, 1, 1, False),
-    ("balanced", "fp32", 64, 2, 1, False),
-    ("balanced", "fp32", 64, 4, 1, False),
-    ("balanced", "fp32", 128, 2, 1, False),
-    ("balanced", "fp32", 128, 4, 1, False),
-    ("balanced", "fp32", 256, 2, 1, False),
-    ("balanced", "fp32", 256, 4, 1, False),
-    ("balanced", "fp32", 512, 2, 1, False),
-    ("balanced", "fp32", 512, 4, 1, False),
-    ("balanced", "fp32", 256, 4, 2, False),
-    ("balanced", "fp32", 1024, 4, 2, False),
-    ("balanced", "fp32", 256, 4, 1, True),
-    ("balanced", "bf16x6", 64, 2, 1, False),
-    ("balanced", "bf16x6", 64, 4, 1, False),
-    ("balanced", "bf16x6", 256, 2, 1, False),
-    ("balanced", "bf16x6", 256, 4, 1, False),
+    ("windows", "fp32", 0, 4, 1, False, 0),
+    ("windows", "fp32", 0, 4, 2, False, 0),
+    ("windows", "fp32", 0, 2, 1, False, 0),
+    ("windows", "bf16x6", 0, 4, 1, False, 0),
+    ("balanced", "fp32", 32, 2, 1, False, 0),
+    ("balanced", "fp32", 64, 1, 1, False, 0),
+    ("balanced", "fp32", 64, 2, 1, False, 0),
+    ("balanced", "fp32", 64, 4, 1, False, 0),
+    ("balanced", "fp32", 128, 2, 1, False, 0),
+    ("balanced", "fp32", 128, 4, 1, False, 0),
+    ("balanced", "fp32", 256, 2, 1, False, 0),
+    ("balanced", "fp32", 256, 4, 1, False, 0),
+    ("balanced", "fp32", 512, 2, 1, False, 0),
+    ("balanced", "fp32", 512, 4, 1, False, 0),
+    ("balanced", "fp32", 256, 4, 2, False, 0),
+    ("balanced", "fp32", 1024, 4, 2, False, 0),
+    ("balanced", "fp32", 256, 4, 1, True, 0),
+    ("balanced", "bf16x6", 64, 2, 1, False, 0),
+    ("balanced", "bf16x6", 64, 4, 1, False, 0),
+    ("balanced", "bf16x6", 256, 2, 1, False, 0),
+    ("balanced", "bf16x6", 256, 4, 1, False, 0),
+    ("balanced", "fp32", 64, 4, 1, False, 256),
+    ("balanced", "fp32", 128, 2, 1, False, 256),
+    ("balanced", "fp32", 128, 4, 1, False, 256),
+    ("balanced", "fp32", 256, 4, 1, False, 512),
 ]
 
 
@@ -85,8 +92,8 @@ def default_variant(num_blocks: int, max_window_blocks: int) -> int:
     no window is longer than AUTO_WINDOW_BLOCKS, else balanced with 32-column
     tiles and the default chunk."""
     if default_schedule(max_window_blocks) == "windows":
-        return VARIANTS.index(("windows", "fp32", 0, 4, 1, False))
-    return VARIANTS.index(("balanced", "fp32", default_chunk_blocks(int(num_blocks)), 2, 1, False))
+        return VARIANTS.index(("windows", "fp32", 0, 4, 1, False, 0))
+    return VARIANTS.index(("balanced", "fp32", default_chunk_blocks(int(num_blocks)), 2, 1, False, 0))
 
 
 def max_window_blocks_of(features_row) -> int:
@@ -124,8 +131,8 @@ def out_of_range(x, lo, hi) -> list:
 
 
 def variant_name(v) -> str:
-    s, p, c, vec, sub, pers = v
-    return f"{s}:{p}:c{c}:v{vec}x{sub}{':persistent' if pers else ''}"
+    s, p, c, vec, sub, pers, span = v
+    return f"{s}:{p}:c{c}:v{vec}x{sub}{':persistent' if pers else ''}{f':s{span}' if span else ''}"
 
 
 # ------------------------------------------------------------------ features
@@ -166,10 +173,10 @@ def time_variants(A, B, variants=VARIANTS, iters: int = 3) -> np.ndarray:
     C = torch.empty((A.num_rows, B.shape[1]), device=B.device, dtype=torch.float32)
     stream = torch.cuda.current_stream()
     out = np.full(len(variants), np.nan)
-    for i, (sched, prec, chunk, vec, sub, pers) in enumerate(variants):
+    for i, (sched, prec, chunk, vec, sub, pers, span) in enumerate(variants):
         def run():
             A.spmm(B, schedule=sched, precision=prec, chunk_blocks=chunk, vec=vec,
-                   subtiles=sub, persistent=pers, out=C)
+                   subtiles=sub, persistent=pers, out=C, cut_span_blocks=span)
         run()
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
@@ -193,15 +200,31 @@ FAMILIES = ["uniform", "rmat", "banded", "block_diagonal", "sbm"]
 WIDTHS = [32, 64, 128, 256, 512]
 
 
+# Seeds [LARGE_START, XL_START): power-law matrices of 2^20..2^21 rows
+# (R-MAT and the block model with hubs, alternating); seeds [XL_START,
+# XL_END): the block model at the size of the 232,965-row configuration
+# (150K..400K rows, 200..520 entries per row, 200 hub rows of 20K..50K).
+LARGE_START, XL_START, XL_END = 2000, 2120, 2140
+
+
 def matrix_spec(seed: int, max_nnz: float = 3.2e7):
     """Family, rows (log-uniform 1e4..1e6), average row length (log-uniform
     4..512, capped so nnz <= max_nnz), N and value presence of training matrix
-    `seed` (deterministic)."""
+    `seed` (deterministic); the large seeds as described above."""
     rng = np.random.default_rng(10_000 + seed)
-    family = FAMILIES[seed % len(FAMILIES)]
-    rows = int(round(10 ** rng.uniform(4.0, 6.0)))
-    avg = 10 ** rng.uniform(math.log10(4), math.log10(512))
-    avg = min(avg, max_nnz / rows)
+    if LARGE_START <= seed < XL_START:
+        family = "rmat" if seed % 2 == 0 else "sbm"
+        rows = int(round(2 ** rng.uniform(20.0, 21.0)))
+        avg = min(10 ** rng.uniform(math.log10(8), math.log10(32)), max_nnz / rows)
+    elif XL_START <= seed < XL_END:
+        family = "sbm"
+        rows = int(round(10 ** rng.uniform(math.log10(1.5e5), math.log10(4e5))))
+        avg = min(10 ** rng.uniform(math.log10(200), math.log10(520)), 1.25e8 / rows)
+    else:
+        family = FAMILIES[seed % len(FAMILIES)]
+        rows = int(round(10 ** rng.uniform(4.0, 6.0)))
+        avg = 10 ** rng.uniform(math.log10(4), math.log10(512))
+        avg = min(avg, max_nnz / rows)
     N = WIDTHS[int(rng.integers(len(WIDTHS)))]
     has_values = bool(rng.random() < 0.5)
     return family, rows, avg, N, has_values, int(rng.integers(1 << 30))
@@ -222,6 +245,9 @@ def make_matrix(seed: int):
     elif family == "block_diagonal":
         blk = int(rng.integers(16, 257))
         a = graphs.block_diagonal(rows, blk, fill=min(1.0, avg / blk), noise=0.01, seed=s)
+    elif XL_START <= seed < XL_END:
+        a = graphs.sbm_hubs(rows, communities=int(rng.integers(30, 71)), avg_per_row=avg,
+                            hubs=200, hub_min=20000, hub_max=50000, seed=s, values=None)
     else:
         a = graphs.sbm_hubs(rows, communities=int(rng.integers(10, 51)), avg_per_row=avg,
                             hubs=max(1, min(200, rows // 5000)), hub_min=max(1, rows // 50),
@@ -238,7 +264,7 @@ def collect(count: int, out: str, start: int = 0, workers: int = 8, iters: int =
     import torch
     from .matrix import DTCMatrix
     feats, times, meta = [], [], []
-    t0 = time.time()
+    t0 = t_last = time.time()
     for k, (seed, family, N, rowptr, col, vals, M, K) in enumerate(
             pool.imap(make_matrix, range(start, start + count), chunksize=1)):
         A = DTCMatrix(rowptr, col, vals, num_rows=M, num_cols=K)
@@ -249,8 +275,9 @@ def collect(count: int, out: str, start: int = 0, workers: int = 8, iters: int =
         times.append(time_variants(A, B, iters=iters))
         meta.append((seed, FAMILIES.index(family), N, M, A.nnz, A.block_count))
         del A, B
-        if (k + 1) % 25 == 0:
-            print(f"[collect] {k + 1}/{count} matrices, {time.time() - t0:.0f}s", flush=True)
+        if (k + 1) % 25 == 0 or time.time() - t_last > 30:
+            t_last = time.time()
+            print(f"[collect] {k + 1}/{count} matrices, {t_last - t0:.0f}s", flush=True)
     pool.close()
     meta = np.asarray(meta, np.int64)
     np.savez_compressed(out, features=np.stack(feats), times_ms=np.stack(times), seed=meta[:, 0],
@@ -401,8 +428,15 @@ def regret(times_ms, chosen) -> np.ndarray:
     return t[np.arange(t.shape[0]), chosen] / best - 1.0
 
 
+def is_train_seed(seed, holdout_from: int = 1600):
+    """Training split: the seeds below holdout_from, and three in four of the
+    large seeds (LARGE_START on; every fourth is held out)."""
+    seed = np.asarray(seed)
+    return (seed < holdout_from) | ((seed >= LARGE_START) & (seed % 4 != 3))
+
+
 def split(data, holdout_from: int = 1600):
-    train = data["seed"] < holdout_from
+    train = is_train_seed(data["seed"], holdout_from)
     return train, ~train
 
 

@@ -68,6 +68,10 @@ typedef struct dtc_spmm_options {
                             per window) instead of float32 atomics, so repeated calls give
                             bitwise identical results; needs the larger workspace that
                             dtc_spmm_balanced_workspace_size_ex reports */
+  int32_t cut_span_blocks; /* balanced schedule: a window of at most this many TC blocks is
+                              never cut (it belongs whole to the chunk holding its first
+                              block); a longer one is cut at the multiples of this span and
+                              its pieces merged.  A multiple of the chunk; 0 = the chunk */
 } dtc_spmm_options_t;
 
 /* ------------------------------------------------------------------ misc */

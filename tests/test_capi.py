@@ -5,6 +5,7 @@ reference's operator names."""
 import ctypes
 import os
 import re
+import subprocess
 
 import pytest
 import torch
@@ -51,7 +52,7 @@ def test_status_strings(lib):
 class Opts(ctypes.Structure):  # dtc_spmm_options_t
     _fields_ = [("precision", ctypes.c_int32), ("vec", ctypes.c_int32), ("subtiles", ctypes.c_int32),
                 ("chunk_blocks", ctypes.c_int32), ("persistent", ctypes.c_int32), ("b_dtype", ctypes.c_int32),
-                ("deterministic", ctypes.c_int32)]
+                ("deterministic", ctypes.c_int32), ("cut_span_blocks", ctypes.c_int32)]
 
 
 class TcaOpts(ctypes.Structure):  # dtc_tca_options_t
@@ -60,8 +61,17 @@ class TcaOpts(ctypes.Structure):  # dtc_tca_options_t
                 ("max_pairs", ctypes.c_int64)]
 
 
-def test_struct_sizes_match_header():
-    assert ctypes.sizeof(Opts) == 28 and ctypes.sizeof(TcaOpts) == 32
+def test_struct_sizes_match_header(tmp_path):
+    # sizes as the C compiler lays the header's structs out
+    src = tmp_path / "sizes.c"
+    src.write_text('#include "dtc.h"\n#include <stdio.h>\nint main(void) {\n'
+                   '  printf("%zu %zu", sizeof(dtc_spmm_options_t), sizeof(dtc_tca_options_t));\n'
+                   '  return 0;\n}\n')
+    exe = tmp_path / "sizes"
+    subprocess.run(["gcc", "-std=c11", f"-I{os.path.dirname(HEADER)}", str(src), "-o", str(exe)], check=True)
+    spmm_size, tca_size = map(int, subprocess.run([str(exe)], capture_output=True, text=True,
+                                                  check=True).stdout.split())
+    assert ctypes.sizeof(Opts) == spmm_size and ctypes.sizeof(TcaOpts) == tca_size
     from dtc.capi import SpmmOptions
     assert [f[0] for f in SpmmOptions._fields_] == [f[0] for f in Opts._fields_]
 
@@ -148,6 +158,12 @@ def test_host_side_argument_errors(lib):
     st = lib.dtc_spmm_balanced(dummy, dummy, dummy, dummy, None, 4, 8, 16, dummy, 16, 8, dummy,
                                ctypes.byref(o), dummy, 1 << 20, None)
     assert st == 1
+    # the cut span is a non-negative multiple of the chunk
+    for span in (-4, 3):
+        o = Opts(0, 0, 0, 2, 0, 0, 0, span)
+        st = lib.dtc_spmm_balanced(dummy, dummy, dummy, dummy, None, 4, 8, 16, dummy, 16, 8, dummy,
+                                   ctypes.byref(o), dummy, 1 << 20, None)
+        assert st == 1 and b"cut_span_blocks" in lib.dtc_last_error()
 
 
 @pytest.fixture(scope="module")

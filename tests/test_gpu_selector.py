@@ -90,7 +90,7 @@ def test_training_matches_oracle_steps(ext):
 
 
 def test_selector_regret_and_gain(ext):
-    """Train on seeds < 1600 with the HIP kernels; on the held-out seeds the
+    """Train on the training seeds with the HIP kernels; on the held-out seeds the
     selector must beat the library's default options (the method without it:
     the "auto" schedule rule) and the best single fixed variant."""
     from dtc.selector import DEFAULT_DATA, MLPSelector, evaluate, split

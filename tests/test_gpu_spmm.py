@@ -386,3 +386,42 @@ def test_duplicate_entries_add(ext, schedule):
     deg = torch.from_numpy(np.diff(rowptr).astype(np.float32)).cuda()
     C = case.run(ext, ones, schedule, "fp32", use_vals=False)
     assert torch.equal(C, deg[:, None].expand_as(C))
+
+
+@pytest.mark.parametrize("persistent", [False, True])
+@pytest.mark.parametrize("deterministic", [False, True])
+def test_cut_span(ext, ragged, gappy, rmat12, persistent, deterministic):
+    """Windows of up to cut_span_blocks TC blocks are kept whole and longer
+    ones are cut at its multiples: every chunk/span pair gives the FP64
+    product (output pre-filled with NaN: every element written), and the
+    deterministic merge stays bitwise repeatable with pieces longer than a
+    chunk."""
+    for case in (ragged, gappy, rmat12):
+        rng = np.random.default_rng(7)
+        N = 40
+        Bn = rng.standard_normal((case.M, N)).astype(np.float32)
+        B = torch.from_numpy(Bn).cuda()
+        C64, Cabs = case.ref(Bn)
+        for chunk in (1, 3, 8, 64):
+            for mult in (1, 2, 5, 16, 64):
+                span = chunk * mult
+                say("cut span", case.name, persistent, deterministic, chunk, span)
+                first = None
+                for _ in range(2 if deterministic else 1):
+                    out = torch.full((case.M, N), float("nan"), device="cuda")
+                    case.run(ext, B, "balanced", "fp32", chunk_blocks=chunk, cut_span_blocks=span,
+                             persistent=persistent, deterministic=deterministic, out=out)
+                    if first is None:
+                        first = out.clone()
+                        C = out.cpu().numpy()
+                        assert np.isfinite(C).all(), (case.name, chunk, span)
+                        assert componentwise(C, C64, Cabs) <= TOL["fp32"], (case.name, chunk, span)
+                    else:
+                        assert torch.equal(out, first), (case.name, chunk, span)
+
+
+def test_cut_span_must_be_a_chunk_multiple(ext, rmat12):
+    B = torch.randn((rmat12.M, 16), device="cuda")
+    say("cut span argument check")
+    with pytest.raises(RuntimeError, match="cut_span_blocks"):
+        rmat12.run(ext, B, "balanced", "fp32", chunk_blocks=8, cut_span_blocks=12)

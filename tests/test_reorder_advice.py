@@ -19,8 +19,11 @@ def data():
 def test_dataset_shapes_and_families(data):
     n = data["features"].shape[0]
     assert data["features"].shape == (n, len(sel.FEATURE_NAMES))
-    assert data["times_orig_ms"].shape == data["times_reordered_ms"].shape == (n, len(sel.VARIANTS))
-    assert [str(s) for s in data["variants"]] == [sel.variant_name(v) for v in sel.VARIANTS]
+    # timed over the selector's variants of the time of collection: a subset
+    # of today's (gains are the best time over whichever were timed)
+    nv = len(data["variants"])
+    assert data["times_orig_ms"].shape == data["times_reordered_ms"].shape == (n, nv)
+    assert set(str(s) for s in data["variants"]) <= set(sel.variant_name(v) for v in sel.VARIANTS)
     assert len(np.unique(data["seed"])) == n
     for split in (data["seed"] < 1600, data["seed"] >= 1600):
         assert set(np.unique(data["family"][split]).tolist()) == set(range(len(sel.FAMILIES)))

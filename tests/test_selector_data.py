@@ -67,23 +67,27 @@ def test_dataset_and_model_consistent():
 
 
 def test_model_range_guard():
-    """The shipped network carries its training feature range; the 232,965-row
-    block model (121M entries, 10.4M TC blocks) lies outside it and goes to
-    exhaustive timing, while the training matrices and all but a few of the
-    400 held-out ones lie inside."""
+    """The shipped network carries its training feature range: the training
+    matrices and all but a few of the held-out ones lie inside it, and so does
+    the 232,965-row block model (121M entries, 10.4M TC blocks, covered by the
+    block-model seeds of that size); a matrix four times larger lies outside
+    and goes to exhaustive timing."""
     from dtc import selector as sel
     z = np.load(sel.DEFAULT_MODEL)
     d = np.load(sel.DEFAULT_DATA)
-    train = d["seed"] < 1600
+    train = sel.is_train_seed(d["seed"])
     X = d["features"].astype(np.float32)
     assert np.array_equal(z["lo"], X[train].min(0)) and np.array_equal(z["hi"], X[train].max(0))
     assert all(sel.out_of_range(x, z["lo"], z["hi"]) == [] for x in X[train])
     assert sum(1 for x in X[~train] if sel.out_of_range(x, z["lo"], z["hi"])) <= 4
-    x = X[0].copy()
-    x[sel.FEATURE_NAMES.index("log2_nnz")] = np.log2(1.21e8)
-    x[sel.FEATURE_NAMES.index("log2_blocks")] = np.log2(1.04e7)
-    assert sel.out_of_range(x, z["lo"], z["hi"]) == ["log2_nnz", "log2_blocks"]
-    assert sel.out_of_range(x, None, None) == []
+    xl = X[d["seed"] == sel.XL_START][0].copy()
+    xl[sel.FEATURE_NAMES.index("log2_nnz")] = np.log2(1.21e8)
+    xl[sel.FEATURE_NAMES.index("log2_blocks")] = np.log2(1.04e7)
+    assert sel.out_of_range(xl, z["lo"], z["hi"]) == []
+    xl[sel.FEATURE_NAMES.index("log2_nnz")] = np.log2(4 * 1.21e8)
+    xl[sel.FEATURE_NAMES.index("log2_blocks")] = np.log2(4 * 1.04e7)
+    assert sel.out_of_range(xl, z["lo"], z["hi"]) == ["log2_nnz", "log2_blocks"]
+    assert sel.out_of_range(xl, None, None) == []
 
 
 def test_default_schedule_rule():
@@ -95,17 +99,17 @@ def test_default_schedule_rule():
     assert sel.default_schedule(0) == "windows"
     assert sel.default_schedule(sel.AUTO_WINDOW_BLOCKS) == "windows"
     assert sel.default_schedule(sel.AUTO_WINDOW_BLOCKS + 1) == "balanced"
-    assert sel.VARIANTS[sel.default_variant(10_000, 12)] == ("windows", "fp32", 0, 4, 1, False)
-    assert sel.VARIANTS[sel.default_variant(10_000_000, 5_000)] == ("balanced", "fp32", 512, 2, 1, False)
+    assert sel.VARIANTS[sel.default_variant(10_000, 12)] == ("windows", "fp32", 0, 4, 1, False, 0)
+    assert sel.VARIANTS[sel.default_variant(10_000_000, 5_000)] == ("balanced", "fp32", 512, 2, 1, False, 0)
     x = np.zeros(len(FEATURE_NAMES), np.float32)
     x[FEATURE_NAMES.index("log2_max_blocks_per_window")] = np.log2(161.0)
     assert sel.max_window_blocks_of(x) == 160
     z = np.load(sel.DEFAULT_DATA)
-    test = z["seed"] >= 1600
+    test = ~sel.is_train_seed(z["seed"])
     t, X, blocks = z["times_ms"][test], z["features"][test], z["blocks"][test]
     best = np.nanmin(t, axis=1)
     rows = np.arange(t.shape[0])
     rule = np.array([sel.default_variant(b, sel.max_window_blocks_of(x)) for b, x in zip(blocks, X)])
-    balanced_only = np.array([sel.VARIANTS.index(("balanced", "fp32", sel.default_chunk_blocks(int(b)), 2, 1, False))
+    balanced_only = np.array([sel.VARIANTS.index(("balanced", "fp32", sel.default_chunk_blocks(int(b)), 2, 1, False, 0))
                               for b in blocks])
     assert (t[rows, rule] / best).mean() < (t[rows, balanced_only] / best).mean()
