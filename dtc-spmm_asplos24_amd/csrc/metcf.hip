@@ -52,7 +52,6 @@ struct MetcfWs {
   uint64_t *keys0, *keys1;
   int32_t *vals0, *vals1;
   int32_t *gscan;    // inclusive unique-column count per sorted element
-  int32_t *uniqpos;  // sorted position of every unique column (+ sentinel)
   void *tmp;         // scan / sort scratch
   size_t bytes;
 };
@@ -66,7 +65,6 @@ MetcfWs carve(void *ws, int64_t nnz, int64_t W) {
   w.vals0 = c.take<int32_t>((size_t)std::max<int64_t>(nnz, 1));
   w.vals1 = c.take<int32_t>((size_t)std::max<int64_t>(nnz, 1));
   w.gscan = c.take<int32_t>((size_t)std::max<int64_t>(nnz, 1));
-  w.uniqpos = c.take<int32_t>((size_t)nnz + 1);
   const size_t tmp = std::max({radix_sort_workspace_bytes(nnz), scan_workspace_bytes(nnz),
                                scan_workspace_bytes(W)});
   w.tmp = c.take<char>(tmp);
@@ -216,8 +214,7 @@ __global__ __launch_bounds__(256) void k_edge_to_column(const int32_t *rowptr, i
 __global__ __launch_bounds__(256) void k_fill_blocks(
     const int32_t *rowptr, const int32_t *rwo, int32_t M, int32_t nnz, int32_t blk_h,
     int32_t blk_w, int32_t T, const MetcfHeader *h, const uint64_t *keys0,
-    const uint64_t *keys1, const int32_t *g, int32_t *uniqpos, int32_t *off, int32_t *rowid,
-    int32_t *atox) {
+    const uint64_t *keys1, const int32_t *g, int32_t *off, int32_t *rowid, int32_t *atox) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nnz) return;
   const int colbits = h->colbits;
@@ -225,10 +222,7 @@ __global__ __launch_bounds__(256) void k_fill_blocks(
   const uint64_t key = keys[i];
   const int gi = g[i];
   const bool is_new = (i == 0) || (g[i - 1] != gi);
-  if (i == 0) {
-    off[T] = nnz;
-    uniqpos[g[nnz - 1]] = nnz;
-  }
+  if (i == 0) off[T] = nnz;
   if (!is_new) return;
   const int w = (int)(key >> colbits);
   const int colv = (int)(key & ((1ull << colbits) - 1));
@@ -238,7 +232,6 @@ __global__ __launch_bounds__(256) void k_fill_blocks(
   const int r = gi - base - 1;
   const int64_t b = (int64_t)rwo[w] + r / blk_w;
   const int c = r % blk_w;
-  uniqpos[gi - 1] = i;
   atox[b * blk_w + c] = colv;
   if (c == 0) {
     off[b] = i;
@@ -252,15 +245,16 @@ __global__ __launch_bounds__(256) void k_fill_blocks(
 // Phase 2b: tile ids (and the optional reordered values / edge ids).  Within
 // a block the reference stores elements in CSR order; here an element's slot
 // is its block offset plus the number of elements of the same block with a
-// smaller CSR index.  Each condensed column's run in the sorted order is
-// ascending in CSR index (the sort is stable), so that count is one binary
-// search per column of the block.
+// smaller CSR index.  A block's elements are one contiguous range of the
+// sorted order, [off[b], off[b+1]) (its columns are consecutive condensed
+// columns of one window), so that count is one pass over the range: at most
+// blk_h * blk_w = 128 loads that the lanes of one block share (the same
+// addresses), from lines the neighbouring lanes have just brought in.
 __global__ __launch_bounds__(256) void k_tile_ids(
     const int32_t *rowptr, const int32_t *rwo, const int32_t *e2r, int32_t M, int32_t nnz,
     int32_t blk_h, int32_t blk_w, const MetcfHeader *h, const uint64_t *keys0,
     const uint64_t *keys1, const int32_t *vals0, const int32_t *vals1, const int32_t *g,
-    const int32_t *uniqpos, const int32_t *off, const float *values, uint8_t *tile,
-    float *tcvalues, int32_t *tcedge) {
+    const int32_t *off, const float *values, uint8_t *tile, float *tcvalues, int32_t *tcedge) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nnz) return;
   const int colbits = h->colbits;
@@ -268,23 +262,17 @@ __global__ __launch_bounds__(256) void k_tile_ids(
   const int32_t *vals = h->sorted_in_1 ? vals1 : vals0;
   const int w = (int)(keys[i] >> colbits);
   const int s = window_begin(rowptr, w, blk_h, M);
-  const int e = window_end(rowptr, w, blk_h, M);
   const int base = s > 0 ? g[s - 1] : 0;
-  const int U = g[e - 1] - base;
   const int r = g[i] - base - 1;
   const int kb = r / blk_w;
   const int c = r % blk_w;
   const int64_t b = (int64_t)rwo[w] + kb;
   const int myv = vals[i];
-  int pos = off[b] + (i - uniqpos[base + r]);
-  for (int cc = 0; cc < blk_w; ++cc) {
-    const int r2 = kb * blk_w + cc;
-    if (cc == c || r2 >= U) continue;
-    const int rs = uniqpos[base + r2];
-    const int re = uniqpos[base + r2 + 1];
-    pos += lower_bound_dev<int32_t>(vals, rs, re, myv) - rs;
-  }
-  if (pos >= off[b + 1]) return;  // unreachable for a consistent build; keeps writes in bounds
+  const int lo = off[b], hi = off[b + 1];
+  int pos = lo;
+#pragma unroll 8
+  for (int j = lo; j < hi; ++j) pos += vals[j] < myv;
+  if (pos >= hi) return;  // unreachable for a consistent build; keeps writes in bounds
   const unsigned row_local = (unsigned)e2r[myv] % (unsigned)blk_h;
   tile[pos] = (uint8_t)(row_local * (unsigned)blk_w + (unsigned)c);
   if (tcvalues) tcvalues[pos] = values ? values[myv] : 1.0f;
@@ -456,12 +444,11 @@ dtc_status_t dtc_metcf_build_blocks(const int32_t *node_pointer, const int32_t *
   const unsigned grid = grid_for(nnz, 256);
   k_fill_blocks<<<grid, 256, 0, stream>>>(node_pointer, row_window_offset, num_nodes, nnz, blk_h,
                                           blk_w, block_count, ws.hdr, ws.keys0, ws.keys1,
-                                          ws.gscan, ws.uniqpos, tcblock_offset, tcblock_rowid,
-                                          sparse_atox);
+                                          ws.gscan, tcblock_offset, tcblock_rowid, sparse_atox);
   DTC_LAUNCH_CHECK("fill_blocks");
   k_tile_ids<<<grid, 256, 0, stream>>>(node_pointer, row_window_offset, edge_to_row, num_nodes,
                                        nnz, blk_h, blk_w, ws.hdr, ws.keys0, ws.keys1, ws.vals0,
-                                       ws.vals1, ws.gscan, ws.uniqpos, tcblock_offset, values,
+                                       ws.vals1, ws.gscan, tcblock_offset, values,
                                        tcblocktile_id, tcblock_values, tcblock_edge);
   DTC_LAUNCH_CHECK("tile_ids");
   return DTC_SUCCESS;

@@ -138,7 +138,7 @@ dtc_status_t scan_key_changes(const uint64_t *keys, int32_t *out, int64_t n, voi
 // -------------------------------------------------------------- radix sort
 
 constexpr int kSortThreads = 256;
-constexpr int kSortItems = 8;
+constexpr int kSortItems = 16;
 constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr int kRadix = 256;
 
@@ -161,67 +161,60 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint64_t
 }
 
 // Stable scatter: element i of the tile goes to offsets[d][tile] + (number of
-// earlier elements of the tile with the same digit d).  Elements are taken in
-// rounds of 256 in index order; inside a round the rank among equal digits is
-// found with eight ballots (one per digit bit) and a popcount.  The tile is
-// first laid out in LDS in digit order (its digit counts are differences of
-// consecutive scanned offsets), then written out so that consecutive threads
-// store consecutive elements of one digit's run: runs of ~16 keys land in
-// consecutive global addresses instead of 4,096 scattered 12-byte writes.
+// earlier elements of the tile with the same digit d).  Tiles are 4,096
+// keys.  Wave w owns the tile's elements [1024 w, 1024 w + 1024), taken in 16
+// rounds of 64 in index order, so index order is wave-major, then round,
+// then lane.  Each wave
+// ranks its own elements without workgroup barriers: inside a round the rank
+// among equal digits comes from eight ballots (one per digit bit) and a
+// popcount, added to the wave's running count of the digit (per-wave LDS
+// counters).  Then, once per tile, the four waves' counts give each digit's
+// per-wave prefixes and the tile's digit-sorted layout (a scan over the 256
+// digits), and the tile is laid out in LDS in digit order and written out so
+// that consecutive threads store consecutive elements of one digit's run
+// (~16 keys on average: runs, not 4,096 scattered 12-byte writes; with
+// 2,048-key tiles the runs of ~8 keys made the scatter 1.5x slower than a
+// pass whose few digits give long runs).
+constexpr int kSortWaves = kSortThreads / 64;
+constexpr int kWaveItems = kSortTile / kSortWaves;  // 1,024 elements per wave
+
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const uint64_t *kin, const int32_t *vin, uint64_t *kout, int32_t *vout, int64_t n,
     int shift, const int32_t *offsets, int ntiles) {
-  __shared__ int s_off[kRadix];   // global position of the tile's first element of digit d
-  __shared__ int s_loc[kRadix];   // local position of the same element
-  __shared__ int s_run[kRadix];   // elements of digit d placed so far
-  __shared__ int s_wcnt[4][kRadix];
+  __shared__ int s_off[kRadix];             // global position of the tile's first element of digit d
+  __shared__ int s_loc[kRadix];             // local position of the same element
+  __shared__ int s_wcnt[kSortWaves][kRadix];  // per-wave digit counts, then per-wave prefixes
+  __shared__ int s_scan[kSortWaves];
   __shared__ uint64_t s_keys[kSortTile];
   __shared__ int32_t s_vals[kSortTile];
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
   const int lane = tid & 63;
   const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-  const int64_t idx = (int64_t)tid * ntiles + blockIdx.x;
-  const int64_t total = (int64_t)kRadix * ntiles;
-  const int off = offsets[idx];
-  const int cnt = (idx + 1 < total ? offsets[idx + 1] : (int32_t)n) - off;  // tile's digit-tid count
-  s_off[tid] = off;
-  s_loc[tid] = cnt;
-  s_run[tid] = 0;
-  __syncthreads();
-  // inclusive scan of the 256 digit counts (Hillis-Steele in LDS), then exclusive
-  for (int d = 1; d < kRadix; d <<= 1) {
-    const int v = tid >= d ? s_loc[tid - d] : 0;
-    __syncthreads();
-    s_loc[tid] += v;
-    __syncthreads();
-  }
-  s_loc[tid] -= cnt;
   const int64_t base = (int64_t)blockIdx.x * kSortTile;
-  // All of the tile's keys and values are loaded up front: the rounds below
-  // are separated by barriers, and a load issued per round would wait out one
-  // memory latency per round.
+  const int64_t wbase = base + (int64_t)wave * kWaveItems;
+  // All of the wave's keys and values are loaded up front (one memory
+  // latency for the whole tile).
   uint64_t kreg[kSortItems];
   int32_t vreg[kSortItems];
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
-    const int64_t i = base + (int64_t)r * kSortThreads + tid;
+    const int64_t i = wbase + r * 64 + lane;
     kreg[r] = i < n ? kin[i] : 0ull;
     vreg[r] = i < n ? vin[i] : 0;
   }
-  __syncthreads();
+  {
+    const int64_t idx = (int64_t)tid * ntiles + blockIdx.x;
+    s_off[tid] = offsets[idx];
+  }
+#pragma unroll
+  for (int k = 0; k < kRadix / 64; ++k) s_wcnt[wave][k * 64 + lane] = 0;
+  wave_lds_sync();
+  int lrank[kSortItems];  // rank among the wave's earlier elements of the same digit
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
-    s_wcnt[0][tid] = 0;
-    s_wcnt[1][tid] = 0;
-    s_wcnt[2][tid] = 0;
-    s_wcnt[3][tid] = 0;
-    __syncthreads();
-    const int64_t i = base + (int64_t)r * kSortThreads + tid;
-    const bool valid = i < n;
-    const uint64_t key = kreg[r];
-    const int32_t val = vreg[r];
-    const int d = (int)((key >> shift) & (kRadix - 1));
+    const bool valid = wbase + r * 64 + lane < n;
+    const int d = (int)((kreg[r] >> shift) & (kRadix - 1));
     uint64_t peers = __ballot(valid);
 #pragma unroll
     for (int bit = 0; bit < 8; ++bit) {
@@ -230,19 +223,39 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
       peers &= b ? m : ~m;
     }
     const int rank = __popcll(peers & lt_mask);
-    if (valid && rank == 0) s_wcnt[wave][d] = __popcll(peers);
-    __syncthreads();
-    if (valid) {
-      int pre = 0;
-      for (int w = 0; w < wave; ++w) pre += s_wcnt[w][d];
-      const int lp = s_loc[d] + s_run[d] + pre + rank;
-      s_keys[lp] = key;
-      s_vals[lp] = val;
-    }
-    __syncthreads();
-    s_run[tid] += s_wcnt[0][tid] + s_wcnt[1][tid] + s_wcnt[2][tid] + s_wcnt[3][tid];
-    __syncthreads();
+    lrank[r] = valid ? s_wcnt[wave][d] + rank : 0;
+    wave_lds_sync();
+    if (valid && rank == 0) s_wcnt[wave][d] += __popcll(peers);
+    wave_lds_sync();
   }
+  __syncthreads();
+  // per digit d = tid: the waves' exclusive prefixes, and the tile's count of d
+  int cnt = 0;
+#pragma unroll
+  for (int w = 0; w < kSortWaves; ++w) {
+    const int c = s_wcnt[w][tid];
+    s_wcnt[w][tid] = cnt;
+    cnt += c;
+  }
+  // exclusive scan of the 256 digit counts: the digit-sorted local layout
+  const int inc = wave_inclusive_sum(cnt);
+  if (lane == 63) s_scan[wave] = inc;
+  __syncthreads();
+  int pre = 0;
+#pragma unroll
+  for (int w = 0; w < kSortWaves; ++w) pre += w < wave ? s_scan[w] : 0;
+  s_loc[tid] = pre + inc - cnt;
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    if (wbase + r * 64 + lane < n) {
+      const int d = (int)((kreg[r] >> shift) & (kRadix - 1));
+      const int lp = s_loc[d] + s_wcnt[wave][d] + lrank[r];
+      s_keys[lp] = kreg[r];
+      s_vals[lp] = vreg[r];
+    }
+  }
+  __syncthreads();
   const int tile_n = (int)(n - base < kSortTile ? n - base : kSortTile);
   for (int j = tid; j < tile_n; j += kSortThreads) {
     const uint64_t key = s_keys[j];
