@@ -92,7 +92,7 @@ def bench_gcn(args):
     import numpy as np
     import torch
 
-    from dtc.gcn import gcn_layer
+    from dtc.gcn import gcn_layer, gcn_two_layer, transform
     from dtc.graphs import rmat
     from dtc.matrix import DTCMatrix, spmm_autograd
     from dtc.metrics import spmm_flops
@@ -120,9 +120,8 @@ def bench_gcn(args):
     W2_32 = torch.from_numpy(rng.uniform(-lim, lim, (F, F)).astype(np.float32)).cuda()
     W2bl, W2_32l = leaf(W2_32.to(torch.bfloat16)), leaf(W2_32)
 
-    def two_layer_bf16():
-        H1 = torch.relu(gcn_layer(vb, Xbl, Wbl, A)).to(torch.bfloat16)
-        return gcn_layer(vb, H1, W2bl, A)
+    def two_layer_bf16():  # the second transform applies the ReLU and the bf16 cast
+        return gcn_two_layer(vb, Xbl, Wbl, W2bl, A)
 
     def two_layer_fp32():
         H1 = torch.relu(spmm_autograd(v32, X32l @ W32l, A))
@@ -143,6 +142,13 @@ def bench_gcn(args):
     vtc = load_extension().gather(v, A.tc_edge)
     runs["spmm_bf16_B"] = lambda: A.spmm(XWb, values_tc=vtc)
     runs["spmm_fp32_B"] = lambda: A.spmm(XW32, values_tc=vtc)
+    # the dense half: libdtc's GCN transform against the torch path it
+    # replaces (bf16 GEMM; ReLU, cast and GEMM of a float32 input)
+    H1f = torch.randn((M, F), device="cuda")
+    runs["transform_bf16_x"] = lambda: transform(Xb, Wb)
+    runs["torch_matmul_bf16_x"] = lambda: torch.matmul(Xb, Wb)
+    runs["transform_relu_f32_x"] = lambda: transform(H1f, Wb, relu=True)
+    runs["torch_relu_cast_matmul_f32_x"] = lambda: torch.matmul(torch.relu(H1f).to(torch.bfloat16), Wb)
     del vals_tc
     out = {}
     for name, fn in runs.items():
@@ -161,6 +167,12 @@ def bench_gcn(args):
         "speedup_bf16_layer_fwd_bwd": round(out["fp32_forward_backward"] / out["bf16_forward_backward"], 3),
         "speedup_bf16_2layer_fwd_bwd": round(out["fp32_2layer_forward_backward"]
                                              / out["bf16_2layer_forward_backward"], 3),
+        # bytes of X read once and B written once (packing W, K x F bf16, included)
+        "transform_roofline": {
+            key: {"bytes": nbytes, "gbps": round(nbytes / out[key] / 1e6, 1),
+                  "frac_of_8tbps": round(nbytes / out[key] / 1e6 / 8000.0, 3)}
+            for key, nbytes in (("transform_bf16_x", M * F * 2 + M * F * 2 + 2 * F * F * 2),
+                                ("transform_relu_f32_x", M * F * 4 + M * F * 2 + 2 * F * F * 2))},
     }), flush=True)
 
 

@@ -23,7 +23,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 INCLUDE = os.path.join(REPO, "include")
 HIP_SOURCES = ["primitives.hip", "metcf.hip", "spmm.hip", "backward.hip", "selector.hip",
-               "reorder.hip"]
+               "reorder.hip", "gcn.hip"]
 HEADERS = [os.path.join(CSRC, "dtc_internal.h"), os.path.join(INCLUDE, "dtc.h")]
 LIB = os.path.join(HERE, "libdtc.so")
 # measurement aid beside the library (not part of include/dtc.h)

@@ -531,6 +531,43 @@ Tensor gather(Tensor values, Tensor index) {
   return out;
 }
 
+// ------------------------------------------------------ GCN feature transform
+
+// W [K x F] bf16 -> the fragment order dtc_gcn_transform reads (K * F bf16).
+Tensor gcn_pack_weights(Tensor W) {
+  CHECK_INPUT(W);
+  CHECK_DTYPE(W, torch::kBFloat16);
+  TORCH_CHECK_VALUE(W.dim() == 2, "W must be 2-D [K x F]");
+  const c10::OptionalDeviceGuard guard(W.device());
+  Tensor out = torch::empty({W.numel()}, W.options());
+  check_status(dtc_gcn_pack_weights(W.data_ptr(), (int32_t)W.size(0), (int32_t)W.size(1), out.data_ptr(),
+                                    current_stream()),
+               "gcn_pack_weights");
+  return out;
+}
+
+// B = bf16(act(X) W): X [M x K] float32 or bf16, W packed by gcn_pack_weights
+// for K x F; act = ReLU when relu.  Returns bf16 [M x F].
+Tensor gcn_transform(Tensor X, Tensor W_packed, int64_t F, bool relu) {
+  CHECK_INPUT(X);
+  CHECK_INPUT(W_packed);
+  CHECK_DTYPE(W_packed, torch::kBFloat16);
+  CHECK_SAME_DEVICE(X, W_packed);
+  TORCH_CHECK_VALUE(X.dim() == 2, "X must be 2-D [M x K]");
+  TORCH_CHECK_VALUE(X.scalar_type() == torch::kFloat32 || X.scalar_type() == torch::kBFloat16,
+                    "X must be float32 or bfloat16");
+  const int64_t M = X.size(0), K = X.size(1);
+  TORCH_CHECK_VALUE(W_packed.numel() == K * F, "packed W must hold K * F elements");
+  TORCH_CHECK_VALUE(M < (int64_t(1) << 31) && K < (int64_t(1) << 31) && F < (int64_t(1) << 31), "sizes too large");
+  const c10::OptionalDeviceGuard guard(X.device());
+  Tensor out = torch::empty({M, F}, X.options().dtype(torch::kBFloat16));
+  const int32_t xdt = X.scalar_type() == torch::kFloat32 ? DTC_GCN_X_FLOAT32 : DTC_GCN_X_BFLOAT16;
+  check_status(dtc_gcn_transform(X.data_ptr(), xdt, relu ? 1 : 0, W_packed.data_ptr(), (int32_t)M, (int32_t)K,
+                                 (int32_t)F, out.data_ptr(), current_stream()),
+               "gcn_transform");
+  return out;
+}
+
 // ------------------------------------------------------------- selector
 
 void check_f32_2d(const Tensor &t, const char *name) {
@@ -811,6 +848,10 @@ PYBIND11_MODULE(DTCSpMM, m) {
   m.def("sddmm_csr", &sddmm_csr, "out[e] = X[row(e)] . Y[col(e)]", py::arg("nodePointer"), py::arg("edgeList"),
         py::arg("X"), py::arg("Y"), py::arg("edgeToRow") = py::none());
   m.def("gather", &gather, "out[p] = values[index[p]]", py::arg("values"), py::arg("index"));
+  m.def("gcn_pack_weights", &gcn_pack_weights, "W [K x F] bf16 -> the GCN transform's fragment order",
+        py::arg("W"));
+  m.def("gcn_transform", &gcn_transform, "bf16(act(X) W) as the bf16 B of the SpMM", py::arg("X"),
+        py::arg("W_packed"), py::arg("F"), py::arg("relu") = false);
   m.attr("BLK_H") = kBlkH;
   m.attr("BLK_W") = kBlkW;
   m.attr("library_version") = dtc_version();

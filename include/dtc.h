@@ -358,6 +358,30 @@ dtc_status_t dtc_tca_reorder(const int32_t *node_pointer, const int32_t *edge_li
                              int32_t *reorder_id, dtc_tca_stats_t *stats, void *workspace,
                              size_t workspace_bytes, dtc_stream_t stream);
 
+/* ------------------------------------------------------- GCN feature transform
+ *
+ * The dense half of a GCN layer H = A (act(X) W): the reference's GCN forward
+ * (DTCSpMM_gcn, DTC-SpMM/DTCSpMM.cpp:147-161 over
+ * DTC-SpMM/DTCSpMM_kernel.cu:1033-1116) takes X W as a float32 tensor its
+ * caller computed with a torch GEMM.  Here B = act(X) W is one MFMA kernel
+ * that applies act (identity or ReLU) and the cast to bf16 while it loads X,
+ * accumulates in float32, and writes B as row-major bf16 [M x F] -- the
+ * operand of the SpMM with b_dtype DTC_B_BFLOAT16.  W (bf16, [K x F]
+ * row-major) is first packed into the kernel's fragment order (K * F bf16,
+ * 16-byte aligned).  K must be a multiple of 32 and F of 64; X, the packed W
+ * and the output 16-byte aligned.
+ */
+typedef enum dtc_gcn_x_dtype {
+  DTC_GCN_X_FLOAT32 = 0, /* X is float32: rounded to bf16 (after act) as it is loaded */
+  DTC_GCN_X_BFLOAT16 = 1 /* X is bf16 */
+} dtc_gcn_x_dtype_t;
+
+dtc_status_t dtc_gcn_pack_weights(const void *w, int32_t k, int32_t f, void *w_packed,
+                                  dtc_stream_t stream);
+/* out[i][j] = bf16( sum_k act(X[i][k]) W[k][j] ), act = ReLU when relu = 1. */
+dtc_status_t dtc_gcn_transform(const void *x, int32_t x_dtype, int32_t relu, const void *w_packed,
+                               int32_t m, int32_t k, int32_t f, void *out, dtc_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -109,7 +109,7 @@ def test_host_side_argument_errors(lib):
     assert lib.dtc_metcf_workspace_size(-1, 10, 16, 8, ctypes.byref(size)) == 1
     assert lib.dtc_metcf_workspace_size(10, 10, 0, 8, ctypes.byref(size)) == 1
     assert lib.dtc_metcf_workspace_size(100, 1000, 16, 8, ctypes.byref(size)) == 0
-    assert size.value > 1000 * 32
+    assert size.value >= 1000 * 28  # two key (8 B) and two value (4 B) arrays and the scan (4 B) per entry
     dummy = ctypes.c_void_p(16)
     # N = 0 is rejected before anything touches the device.
     st = lib.dtc_spmm_windows(dummy, dummy, dummy, dummy, None, 1, 1, 4, 16, dummy, 16, 0, dummy, None, None)
