@@ -436,6 +436,68 @@ std::vector<Tensor> DTCSpMM_gcn(Tensor input, Tensor Rowwindow_offset, Tensor TC
   return run_windows(o, Rowwindow_offset, num_nodes, num_edges, opt, false, true, c10::nullopt);
 }
 
+// The reference binds its backward names to the forward functions
+// (DTCSpMM.cpp:386-393): called with dY they return A dY, the gradient with
+// respect to B only when A is symmetric.  Here they also take, as an optional
+// last argument `transposed`, A^T's ME-TCF operands -- (Rowwindow_offset,
+// TCblocktile_id, TCblock_offset, sparse_AToX_idx, values or None, rows of
+// A^T) for the window schedules, TCblock_rowid in place of Rowwindow_offset
+// for the balanced one (dtc.matrix.DTCMatrix.transposed_operands builds
+// them) -- and with it return the true gradient dB = A^T dY.  Without it they
+// behave as the reference's.
+struct TransposedOperands {
+  Tensor index, tile, off, atox;
+  c10::optional<Tensor> values;
+  int64_t rows;
+};
+
+TransposedOperands unpack_transposed(const py::tuple &t) {
+  TORCH_CHECK_VALUE(t.size() == 6,
+                    "transposed must be (index, TCblocktile_id, TCblock_offset, sparse_AToX_idx, values or None, "
+                    "num_rows) of A^T");
+  TransposedOperands o{t[0].cast<Tensor>(), t[1].cast<Tensor>(), t[2].cast<Tensor>(), t[3].cast<Tensor>(),
+                       c10::nullopt, t[5].cast<int64_t>()};
+  if (!t[4].is_none()) o.values = t[4].cast<Tensor>();
+  return o;
+}
+
+std::vector<Tensor> backward_windows(Tensor input, Tensor Rowwindow_offset, Tensor TCblocktile_id,
+                                     Tensor TCblock_offset, Tensor sparse_AToX_idx, int64_t num_nodes,
+                                     int64_t num_edges, std::string exeplan, c10::optional<Tensor> values,
+                                     std::string precision, c10::optional<py::tuple> transposed) {
+  if (!transposed)
+    return run_DTCSpMM(input, Rowwindow_offset, TCblocktile_id, TCblock_offset, sparse_AToX_idx, num_nodes,
+                       num_edges, exeplan, values, precision);
+  const TransposedOperands t = unpack_transposed(*transposed);
+  return run_DTCSpMM(input, t.index, t.tile, t.off, t.atox, t.rows, t.tile.numel(), exeplan, t.values,
+                     precision);
+}
+
+std::vector<Tensor> backward_windows_gcn(Tensor input, Tensor Rowwindow_offset, Tensor TCblocktile_id,
+                                         Tensor TCblock_offset, Tensor sparse_AToX_idx, int64_t num_nodes,
+                                         int64_t num_edges, c10::optional<Tensor> values,
+                                         c10::optional<py::tuple> transposed) {
+  if (!transposed)
+    return DTCSpMM_gcn(input, Rowwindow_offset, TCblocktile_id, TCblock_offset, sparse_AToX_idx, num_nodes,
+                       num_edges, values);
+  const TransposedOperands t = unpack_transposed(*transposed);
+  return DTCSpMM_gcn(input, t.index, t.tile, t.off, t.atox, t.rows, t.tile.numel(), t.values);
+}
+
+std::vector<Tensor> backward_balanced(Tensor input, Tensor TCblock_rowid, Tensor TCblocktile_id,
+                                      Tensor TCblock_offset, Tensor sparse_AToX_idx, int64_t num_nodes,
+                                      std::string exeplan, c10::optional<Tensor> values, std::string precision,
+                                      int64_t chunk_blocks, bool persistent, bool deterministic,
+                                      int64_t cut_span_blocks, c10::optional<py::tuple> transposed) {
+  if (!transposed)
+    return run_DTCSpMM_balance(input, TCblock_rowid, TCblocktile_id, TCblock_offset, sparse_AToX_idx, num_nodes,
+                               exeplan, values, precision, chunk_blocks, persistent, deterministic,
+                               cut_span_blocks);
+  const TransposedOperands t = unpack_transposed(*transposed);
+  return run_DTCSpMM_balance(input, t.index, t.tile, t.off, t.atox, t.rows, exeplan, t.values, precision,
+                             chunk_blocks, persistent, deterministic, cut_span_blocks);
+}
+
 // General entry (build extension): either schedule, every tuning knob, an
 // optional preallocated output, no CSV side effects.
 Tensor spmm(Tensor input, Tensor window_or_rowid, Tensor TCblocktile_id, Tensor TCblock_offset, Tensor sparse_AToX_idx,
@@ -823,21 +885,23 @@ PYBIND11_MODULE(DTCSpMM, m) {
         py::arg("Rowwindow_offset"), py::arg("TCblocktile_id"), py::arg("TCblock_offset"), py::arg("sparse_AToX_idx"),
         py::arg("num_nodes"), py::arg("num_edges"), py::arg("values") = py::none());
   // The reference binds these names to the forward functions
-  // (DTCSpMM.cpp:386-393); kept as the same aliases.  True gradients are
-  // dtc.matrix.spmm_autograd (dB through the transposed ME-TCF, dvalues by
-  // SDDMM).
-  m.def("backward_ptx_uint8_improved", &run_DTCSpMM, "alias of run_DTCSpMM (reference behaviour)", py::arg("input"),
+  // (DTCSpMM.cpp:386-393): without `transposed` they are the same aliases;
+  // with A^T's operands they return the true gradient A^T dY.
+  m.def("backward_ptx_uint8_improved", &backward_windows,
+        "alias of run_DTCSpMM (reference behaviour); with transposed=A^T's operands, A^T dY", py::arg("input"),
         py::arg("Rowwindow_offset"), py::arg("TCblocktile_id"), py::arg("TCblock_offset"), py::arg("sparse_AToX_idx"),
         py::arg("num_nodes"), py::arg("num_edges"), py::arg("exeplan"), py::arg("values") = py::none(),
-        py::arg("precision") = "fp32");
-  m.def("backward_ptx_uint8_improved_gcn", &DTCSpMM_gcn, "alias of DTCSpMM_gcn (reference behaviour)",
-        py::arg("input"), py::arg("Rowwindow_offset"), py::arg("TCblocktile_id"), py::arg("TCblock_offset"),
-        py::arg("sparse_AToX_idx"), py::arg("num_nodes"), py::arg("num_edges"), py::arg("values") = py::none());
-  m.def("balance_backward_ptx_uint8_prefetch", &run_DTCSpMM_balance, "alias of run_DTCSpMM_balance",
+        py::arg("precision") = "fp32", py::arg("transposed") = py::none());
+  m.def("backward_ptx_uint8_improved_gcn", &backward_windows_gcn,
+        "alias of DTCSpMM_gcn (reference behaviour); with transposed=A^T's operands, A^T dY", py::arg("input"),
+        py::arg("Rowwindow_offset"), py::arg("TCblocktile_id"), py::arg("TCblock_offset"), py::arg("sparse_AToX_idx"),
+        py::arg("num_nodes"), py::arg("num_edges"), py::arg("values") = py::none(), py::arg("transposed") = py::none());
+  m.def("balance_backward_ptx_uint8_prefetch", &backward_balanced,
+        "alias of run_DTCSpMM_balance (reference behaviour); with transposed=A^T's operands, A^T dY",
         py::arg("input"), py::arg("TCblock_rowid"), py::arg("TCblocktile_id"), py::arg("TCblock_offset"),
         py::arg("sparse_AToX_idx"), py::arg("num_nodes"), py::arg("exeplan"), py::arg("values") = py::none(),
         py::arg("precision") = "fp32", py::arg("chunk_blocks") = 0, py::arg("persistent") = false,
-        py::arg("deterministic") = false, py::arg("cut_span_blocks") = 0);
+        py::arg("deterministic") = false, py::arg("cut_span_blocks") = 0, py::arg("transposed") = py::none());
   m.def("spmm", &spmm, "SpMM with explicit schedule and tuning knobs", py::arg("input"), py::arg("window_or_rowid"),
         py::arg("TCblocktile_id"), py::arg("TCblock_offset"), py::arg("sparse_AToX_idx"), py::arg("num_nodes"),
         py::arg("schedule") = "balanced", py::arg("values") = py::none(), py::arg("precision") = "fp32",

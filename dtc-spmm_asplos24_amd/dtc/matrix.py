@@ -123,6 +123,21 @@ class DTCMatrix:
         self.transpose()
         return self._perm_t
 
+    def transposed_operands(self, schedule: str = "windows", values: torch.Tensor | None = None) -> tuple:
+        """A^T's ME-TCF operands as the reference's backward names take them in
+        ``transposed`` (``backward_ptx_uint8_improved``, ``..._gcn``: schedule
+        "windows"; ``balance_backward_ptx_uint8_prefetch``: "balanced"), so that
+        they return dB = A^T dY: (Rowwindow_offset or TCblock_rowid,
+        TCblocktile_id, TCblock_offset, sparse_AToX_idx, A^T's values in its
+        TC-block order or None for all ones, rows of A^T).  ``values``: A's CSR
+        values (default: this matrix's)."""
+        ext = load_extension()
+        At = self.transpose()
+        vals = values if values is not None else self.values
+        vals_t = None if vals is None else ext.gather(ext.gather(vals.contiguous(), self.perm_t), At.tc_edge)
+        index = At.row_window_offset if schedule in ("windows", "unbalanced") else At.tcblock_rowid
+        return (index, At.tcblocktile_id, At.tcblock_offset, At.sparse_atox, vals_t, At.num_rows)
+
 
 class _SpMMFunction(torch.autograd.Function):
     @staticmethod

@@ -186,3 +186,59 @@ def test_autograd_duplicate_entries(ext, schedule):
     assert np.allclose(C.detach().cpu().numpy(), C64, rtol=1e-4, atol=1e-5)
     assert np.allclose(v.grad.cpu().numpy(), gv, rtol=1e-4, atol=1e-4)
     assert np.allclose(Bt.grad.cpu().numpy(), gB, rtol=1e-4, atol=1e-5)
+
+
+def test_reference_backward_names(ext):
+    """The reference's backward names (DTCSpMM.cpp:386-393) on a non-symmetric
+    matrix with values: without `transposed` they return A dY, bitwise the
+    forward call's result, as the reference does; with A^T's operands
+    (DTCMatrix.transposed_operands) they return the true gradient
+    dB = A^T dY, against the float64 product with the transposed CSR (float32
+    tolerance 4e-6 componentwise)."""
+    from dtc.matrix import DTCMatrix
+    rng = np.random.default_rng(41)
+    M = 900
+    rowptr, col = random_csr(rng, M, M, 8.0, sorted_rows=True, empty_frac=0.1)
+    rows = [list(col[rowptr[r]:rowptr[r + 1]]) for r in range(M)]
+    rows[5] = sorted(set(rows[5]) | set(range(0, M, 2)))  # a long row: windows of many blocks
+    rowptr = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int32)
+    col = np.concatenate([np.array(r, np.int32) for r in rows]).astype(np.int32)
+    vals = rng.uniform(-1, 1, col.shape[0]).astype(np.float32)
+    A = DTCMatrix(rowptr, col, vals, num_rows=M, num_cols=M)
+    dYn = rng.standard_normal((M, 48)).astype(np.float32)
+    dY = torch.from_numpy(dYn).cuda()
+    # float64 A^T dY through the transposed CSR
+    rpt, ct, perm = oracle.csr_transpose(rowptr, col, M, M)
+    want, wabs = oracle.spmm_csr_f64(rpt, ct, vals[perm], dYn, M, with_abs=True)
+
+    def comp(C):
+        C = C.cpu().numpy().astype(np.float64)
+        mask = wabs > 0
+        return float((np.abs(C - want)[mask] / wabs[mask]).max())
+
+    nnz = A.nnz
+    w_args = (A.row_window_offset, A.tcblocktile_id, A.tcblock_offset, A.sparse_atox, M, nnz)
+    b_args = (A.tcblock_rowid, A.tcblocktile_id, A.tcblock_offset, A.sparse_atox, M)
+    say("backward names: aliases")
+    (fwd,) = ext.run_DTCSpMM(dY, *w_args, "float4_split", values=A.values_tc)
+    (alias,) = ext.backward_ptx_uint8_improved(dY, *w_args, "float4_split", values=A.values_tc)
+    assert torch.equal(fwd, alias)
+    (fwd_b,) = ext.run_DTCSpMM_balance(dY, *b_args, "float4_split", values=A.values_tc, chunk_blocks=8,
+                                       deterministic=True)
+    (alias_b,) = ext.balance_backward_ptx_uint8_prefetch(dY, *b_args, "float4_split", values=A.values_tc,
+                                                         chunk_blocks=8, deterministic=True)
+    assert torch.equal(fwd_b, alias_b)
+    say("backward names: true gradients")
+    tw = A.transposed_operands("windows")
+    tb = A.transposed_operands("balanced")
+    (g1,) = ext.backward_ptx_uint8_improved(dY, *w_args, "float4_split", values=A.values_tc, transposed=tw)
+    (g2,) = ext.backward_ptx_uint8_improved_gcn(dY, *w_args, values=A.values_tc, transposed=tw)
+    (g3,) = ext.balance_backward_ptx_uint8_prefetch(dY, *b_args, "float4_split", values=A.values_tc,
+                                                    chunk_blocks=8, transposed=tb)
+    for g in (g1, g2, g3):
+        assert g.shape == (M, 48)
+        assert comp(g) <= 4e-6
+    # the forward product is not the gradient here (A is not symmetric)
+    assert comp(fwd) > 1e-2
+    with pytest.raises(ValueError, match="transposed"):
+        ext.backward_ptx_uint8_improved(dY, *w_args, "float4_split", transposed=(tw[0],))
