@@ -145,7 +145,8 @@ def bench_gcn(args):
     # the dense half: libdtc's GCN transform against the torch path it
     # replaces (bf16 GEMM; ReLU, cast and GEMM of a float32 input)
     H1f = torch.randn((M, F), device="cuda")
-    runs["transform_bf16_x"] = lambda: transform(Xb, Wb)
+    ext_ = load_extension()
+    runs["transform_bf16_x"] = lambda: ext_.gcn_transform(Xb, ext_.gcn_pack_weights(Wb), F, False)
     runs["torch_matmul_bf16_x"] = lambda: torch.matmul(Xb, Wb)
     runs["transform_relu_f32_x"] = lambda: transform(H1f, Wb, relu=True)
     runs["torch_relu_cast_matmul_f32_x"] = lambda: torch.matmul(torch.relu(H1f).to(torch.bfloat16), Wb)
