@@ -152,9 +152,13 @@ __device__ __forceinline__ int wave_lower_bound(const T *a, int lo, int hi, T ke
 // its first edge by one search, then takes up to 64 edges per step, each
 // lane locating its edge's row among the next 64 row ends held in lanes.
 // Must be called by whole waves with the same arguments.
-template <class Emit>
-__device__ __forceinline__ void for_each_edge_row(const int32_t *rowptr, int32_t M, int32_t nnz,
-                                                  Emit emit) {
+// As for_each_edge_row, with a per-edge value loaded at the start of each
+// step together with the row ends (one memory round trip per step instead of
+// two): load(e) for the step's edge e = p + lane (clamped to the range), and
+// emit(idx, row, value) gets the value of its own edge.
+template <class Load, class Emit>
+__device__ __forceinline__ void for_each_edge_row_load(const int32_t *rowptr, int32_t M, int32_t nnz,
+                                                       Load load, Emit emit) {
   const int lane = lane_id();
   const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
   const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave_uniform(threadIdx.x >> 6);
@@ -165,6 +169,7 @@ __device__ __forceinline__ void for_each_edge_row(const int32_t *rowptr, int32_t
   int r = wave_uniform(wave_lower_bound<int32_t>(rowptr, 0, M + 1, e_begin + 1) - 1);
   int p = e_begin;
   while (p < e_end) {
+    const auto val = load(min(p + lane, e_end - 1));
     const int64_t bi = (int64_t)r + 1 + lane;
     const int bnd = bi <= M ? rowptr[bi] : INT32_MAX;
     const uint64_t past = __ballot(bnd > p);
@@ -182,10 +187,17 @@ __device__ __forceinline__ void for_each_edge_row(const int32_t *rowptr, int32_t
       else lo = mid + 1;
     }
     const int row = r + lo;
-    if (lane < cnt) emit(idx, row);
+    if (lane < cnt) emit(idx, row, val);  // idx = p + lane here: val is its own edge's
     r = wave_uniform(__shfl(row, cnt - 1, kWave));  // row of the step's last edge
     p += cnt;
   }
+}
+
+template <class Emit>
+__device__ __forceinline__ void for_each_edge_row(const int32_t *rowptr, int32_t M, int32_t nnz,
+                                                  Emit emit) {
+  for_each_edge_row_load(
+      rowptr, M, nnz, [](int) { return 0; }, [&](int idx, int row, int) { emit(idx, row); });
 }
 
 // ----------------------------------------------------- device-wide pieces

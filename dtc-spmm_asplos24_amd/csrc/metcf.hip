@@ -141,16 +141,18 @@ __global__ __launch_bounds__(256) void k_validate(const int32_t *rowptr, const i
 // the last of those ends, and its row is r + (the first j whose end exceeds
 // it), found by a 6-step binary search over the lanes.  A hub row is spread
 // over many wavefronts and short rows share one step, so no wavefront walks a
-// long row serially.
+// long row serially.  The step's columns are loaded with its row ends.
 __global__ __launch_bounds__(256) void k_rows_keys(const int32_t *rowptr, const int32_t *col,
                                                    int32_t M, int32_t nnz, int32_t blk_h,
                                                    int colbits, int32_t *e2r, uint64_t *keys,
                                                    int32_t *vals) {
-  for_each_edge_row(rowptr, M, nnz, [&](int idx, int row) {
-    e2r[idx] = row;
-    keys[idx] = ((uint64_t)(row / blk_h) << colbits) | (uint32_t)col[idx];
-    vals[idx] = idx;
-  });
+  for_each_edge_row_load(
+      rowptr, M, nnz, [&](int e) { return col[e]; },
+      [&](int idx, int row, int c) {
+        e2r[idx] = row;
+        keys[idx] = ((uint64_t)(row / blk_h) << colbits) | (uint32_t)c;
+        vals[idx] = idx;
+      });
 }
 
 __device__ __forceinline__ int window_begin(const int32_t *rowptr, int w, int blk_h, int M) {

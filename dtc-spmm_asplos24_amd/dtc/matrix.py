@@ -41,12 +41,15 @@ class DTCMatrix:
         self.block_partition = torch.zeros(W, dtype=torch.int32, device=self.device)
         self.edge_to_column = torch.zeros(self.nnz, dtype=torch.int32, device=self.device)
         self.edge_to_row = torch.zeros(self.nnz, dtype=torch.int32, device=self.device)
+        # the values are reordered into TC-block order by the build itself (the
+        # tile-id pass), not by a separate gather over tc_edge
         out = ext.preprocess_gpu(self.col, self.rowptr, self.num_rows, BLK_H, BLK_W,
                                  self.block_partition, self.edge_to_column, self.edge_to_row,
-                                 values=None, return_edge_order=True)
+                                 values=self.values, return_edge_order=True)
         (self.row_window_offset, self.tcblock_rowid, self.tcblocktile_id, self.tcblock_offset,
-         self.sparse_atox, self.block_count, self.tc_edge) = out
-        self.values_tc = None if self.values is None else ext.gather(self.values, self.tc_edge)
+         self.sparse_atox, self.block_count) = out[:6]
+        self.values_tc = out[6] if self.values is not None else None
+        self.tc_edge = out[-1]
         self._reset_caches()
 
     def _reset_caches(self) -> None:
