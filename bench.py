@@ -382,15 +382,20 @@ def main():
                               subtiles=sub, persistent=pers, out=C, cut_span_blocks=span)
 
     # ---- the learned selector's pick
+    # the network's first choice, and the fastest of its top TOP_K timed on
+    # this matrix (the selector's pick, dtc.selector.pick_variant)
     feats = sel.features(A, N)
-    pick = None
+    pick = pick1 = None
     ood = []
     if os.path.exists(sel.DEFAULT_MODEL):
         model = sel.MLPSelector.load(sel.DEFAULT_MODEL)
-        pick = int(model.choose(feats[None, :])[0])
         ood = model.out_of_range(feats)
-        log(f"selector pick: {sel.variant_name(sel.VARIANTS[pick])}"
-            + (f" (outside its training range: {', '.join(ood)})" if ood else ""))
+        pick1 = int(model.choose(feats[None, :])[0])
+        if not ood and args.variant == "selector":  # the timing runs only when it decides
+            pick, _ = sel.pick_variant(A, B, model)
+        log(f"selector: network's choice {sel.variant_name(sel.VARIANTS[pick1])}, "
+            + (f"pick {sel.variant_name(sel.VARIANTS[pick])} (top {sel.TOP_K} timed)" if pick is not None
+               else f"outside its training range: {', '.join(ood)}" if ood else "no timed pick (fixed variant)"))
 
     # ---- exhaustive table (the selector's yardstick and fallback)
     table_variants = list(sel.VARIANTS)
@@ -431,7 +436,7 @@ def main():
 
     if args.variant == "selector" and pick is not None and not ood:
         chosen_v = sel.VARIANTS[pick]
-        how = "selector"
+        how = f"selector (top {sel.TOP_K} timed)"
     elif args.variant in ("selector", "exhaustive"):
         # exhaustive timing: the selector's fallback (no model, or a matrix
         # outside its training range) and the explicit choice
@@ -467,8 +472,8 @@ def main():
     if args.workload == "rmat" and args.scale == 20 and N == 256 and os.path.exists(args.traffic_file):
         with open(args.traffic_file) as f:
             tj = json.load(f)
-        if tj.get("variant") == sel.variant_name(chosen_v) and tj.get("source_sha16") == spmm_source_sha16():
-            traffic = tj.get("hbm_bytes_per_launch")
+        if tj.get("source_sha16") == spmm_source_sha16():
+            traffic = tj.get("variants", {}).get(sel.variant_name(chosen_v), {}).get("hbm_bytes_per_launch")
 
     # ---- plain FP32 path: fastest FP32 variant, timed like the chosen one
     fp32 = None
@@ -503,9 +508,13 @@ def main():
     if table:
         best = min(range(len(sel.VARIANTS)), key=lambda i: table[i]["ms"])  # the selector's variants
         selector_info = {"pick": None if pick is None else sel.variant_name(sel.VARIANTS[pick]),
+                         "how": f"network's top {sel.TOP_K} timed on this matrix",
                          "out_of_training_range": ood,
                          "exhaustive_best": sel.variant_name(sel.VARIANTS[best]),
-                         "regret": None if pick is None else round(table[pick]["ms"] / table[best]["ms"] - 1, 4)}
+                         "regret": None if pick is None else round(table[pick]["ms"] / table[best]["ms"] - 1, 4),
+                         "network_choice": None if pick1 is None else sel.variant_name(sel.VARIANTS[pick1]),
+                         "network_choice_regret": (None if pick1 is None
+                                                   else round(table[pick1]["ms"] / table[best]["ms"] - 1, 4))}
 
     # ---- CPU baseline: the oracle's ME-TCF SpMM port on the host cores
     cpu = None

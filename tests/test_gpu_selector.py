@@ -107,3 +107,28 @@ def test_selector_regret_and_gain(ext):
     assert rep["selector_mean_regret"] < rep["default_mean_regret"]
     assert rep["selector_mean_regret"] < rep["best_fixed_mean_regret"]
     assert rep["mean_speedup_over_best_fixed"] > 1.0
+    # timing the network's top choices on the matrix itself only removes regret
+    assert rep["top3_timed_mean_regret"] <= rep["selector_mean_regret"]
+
+
+def test_pick_variant_times_the_top_choices(ext):
+    """dtc.selector.pick_variant on a matrix inside the training range times
+    the network's top 3 predictions and returns the fastest of them; with
+    top_k=1 it returns the network's first choice."""
+    from dtc.matrix import DTCMatrix
+    from dtc.selector import DEFAULT_MODEL, MLPSelector, TOP_K, features, make_matrix, pick_variant
+    if not os.path.exists(DEFAULT_MODEL):
+        pytest.skip("no committed selector model")
+    model = MLPSelector.load(DEFAULT_MODEL)
+    # a training matrix (seed 6: R-MAT), inside the training range by construction
+    _, family, N, rowptr, col, vals, M, K = make_matrix(6)
+    A = DTCMatrix(rowptr, col, vals, num_rows=M, num_cols=K)
+    B = torch.randn((K, N), device="cuda")
+    x = features(A, N)
+    assert model.out_of_range(x) == [], family
+    say("selector pick_variant")
+    top = [int(i) for i in model.top_k(x[None, :], TOP_K)[0]]
+    assert top[0] == int(model.choose(x[None, :])[0]) and len(set(top)) == TOP_K
+    pick, how = pick_variant(A, B, model)
+    assert pick in top and "top" in how
+    assert pick_variant(A, B, model, top_k=1) == (top[0], "selector")
