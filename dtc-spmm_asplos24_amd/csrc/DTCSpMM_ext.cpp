@@ -216,29 +216,30 @@ py::tuple preprocess_gpu(Tensor edgeList, Tensor nodePointer, int64_t num_nodes,
                "preprocess_gpu");
   Tensor ws = torch::empty({(int64_t)ws_bytes}, u8);
   Tensor rwo = torch::empty({W + 1}, i32);
+  Tensor tile = torch::empty({nnz}, u8);
+  Tensor tcv, tce;
+  if (values.has_value()) tcv = torch::empty({nnz}, f32);
+  if (return_edge_order) tce = torch::empty({nnz}, i32);
   dtc_stream_t s = current_stream();
   int32_t T = 0;
   check_status(dtc_metcf_build_windows(edgeList.data_ptr<int32_t>(), nodePointer.data_ptr<int32_t>(),
                                        (int32_t)num_nodes, (int32_t)nnz, (int32_t)blockSize_h,
-                                       (int32_t)blockSize_w, blockPartition.data_ptr<int32_t>(),
+                                       (int32_t)blockSize_w,
+                                       values.has_value() ? values->data_ptr<float>() : nullptr,
+                                       blockPartition.data_ptr<int32_t>(),
                                        edgeToColumn.data_ptr<int32_t>(), edgeToRow.data_ptr<int32_t>(),
-                                       rwo.data_ptr<int32_t>(), &T, ws.data_ptr(), ws_bytes, s),
+                                       rwo.data_ptr<int32_t>(), tile.data_ptr<uint8_t>(),
+                                       values.has_value() ? tcv.data_ptr<float>() : nullptr,
+                                       return_edge_order ? tce.data_ptr<int32_t>() : nullptr, &T,
+                                       ws.data_ptr(), ws_bytes, s),
                "preprocess_gpu");
   Tensor rowid = torch::empty({(int64_t)T}, i32);
-  Tensor tile = torch::empty({nnz}, u8);
   Tensor off = torch::empty({(int64_t)T + 1}, i32);
   Tensor atox = torch::empty({(int64_t)T * blockSize_w}, i32);
-  Tensor tcv, tce;
-  if (values.has_value()) tcv = torch::empty({nnz}, f32);
-  if (return_edge_order) tce = torch::empty({nnz}, i32);
-  check_status(dtc_metcf_build_blocks(nodePointer.data_ptr<int32_t>(), edgeToRow.data_ptr<int32_t>(),
-                                      rwo.data_ptr<int32_t>(), (int32_t)num_nodes, (int32_t)nnz,
-                                      (int32_t)blockSize_h, (int32_t)blockSize_w, T,
-                                      values.has_value() ? values->data_ptr<float>() : nullptr,
-                                      rowid.data_ptr<int32_t>(), tile.data_ptr<uint8_t>(),
+  check_status(dtc_metcf_build_blocks(nodePointer.data_ptr<int32_t>(), rwo.data_ptr<int32_t>(),
+                                      (int32_t)num_nodes, (int32_t)nnz, (int32_t)blockSize_h,
+                                      (int32_t)blockSize_w, T, rowid.data_ptr<int32_t>(),
                                       off.data_ptr<int32_t>(), atox.data_ptr<int32_t>(),
-                                      values.has_value() ? tcv.data_ptr<float>() : nullptr,
-                                      return_edge_order ? tce.data_ptr<int32_t>() : nullptr,
                                       ws.data_ptr(), ws_bytes, s),
                "preprocess_gpu");
   if (env_flag("DTC_VERBOSE"))

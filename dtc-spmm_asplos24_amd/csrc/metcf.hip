@@ -3,27 +3,38 @@
 // The reference builds the format with a thrust sort of (window, column)
 // pairs followed by two kernels that run ONE THREAD PER ROW WINDOW
 // (generate_edgetocolumn, DTCSpMM_kernel.cu:205-249, and
-// generate_tcoffset_id_atob, :252-335).  Here every step is data-parallel
-// over edges:
+// generate_tcoffset_id_atob, :252-335).  The CSR is already grouped by
+// window, so the sort only has to order each window's own entries by column:
+// the sorted positions of window w are exactly its CSR range
+// [rowptr[w*blk_h], rowptr[min(w*blk_h+blk_h, M)]).  Windows are classified
+// by entry count E:
 //
-//   phase 1  check the CSR, then key every edge by (window, column) and sort
-//            the keys stably with the radix sort (value = CSR edge index);
-//            the inclusive scan G of "key differs from its predecessor"
-//            gives every sorted element its global unique-column number, so
-//            the rank of a column inside its window is G - G(window start),
-//            |S_w| = G(window end) - G(window start), blockPartition and the
-//            window offsets follow with one more scan;
-//   phase 2  one thread per sorted element writes SparseAToX, the block
-//            offsets (the first element of a block sits exactly at the
-//            block's offset in the sorted order), the block row ids and
-//            finally the tile ids at offset + (number of elements of the same
-//            block with a smaller CSR index), which reproduces the
-//            reference's CSR-order placement inside a block (:291-300).
+//   LDS path (16-row windows of 8-column blocks, E <= 16,384): one workgroup
+//   per window (five shapes, 1 to 16 waves, up to 512 ... 16,384 entries)
+//   sorts the window's (column, local index) pairs in LDS (LSD radix sort
+//   over the window's column span, 8-bit digits, ranks from LDS atomics in
+//   the first pass and from wave ballots in the stable later ones), numbers
+//   the distinct columns by a workgroup scan and writes every output sized
+//   by the CSR: edgeToRow, edgeToColumn, blockPartition, and the tile ids,
+//   values and edge indices in TC-block order, each entry at its block's
+//   start plus the number of the block's entries with a smaller CSR index
+//   (the reference's CSR-order placement inside a block, :291-300).
 //
-// Sorted positions of window w are exactly its CSR range
-// [rowptr[w*blk_h], rowptr[min(w*blk_h+blk_h, M)]) because the sort key
-// starts with the window.
+//   Global path (larger windows, and every window of other block shapes):
+//   the windows are compacted, keyed (compacted window, column) and sorted
+//   by the device-wide radix sort; the inclusive scan G of "key differs from
+//   its predecessor" gives every sorted element its unique-column number, so
+//   the rank of a column inside its window is G - G(window start), and
+//   edge-parallel kernels write the same outputs.
+//
+// Both paths leave per-window block data in the workspace (distinct columns,
+// block starts, |S_w|); after the caller has allocated the block-sized
+// arrays, phase 2 writes TCblockoffset, TCblockRowid and SparseAToX from it.
+// Both paths write every output of a window and nothing of another window,
+// so the classification changes no result.
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 #include <string>
 
 #include "dtc_internal.h"
@@ -31,14 +42,33 @@
 namespace dtc {
 namespace {
 
+// Window classes of the LDS path: (waves, entries per lane), capacity
+// 64 * waves * entries.
+constexpr int kWinH = 16;  // rows per window on the LDS path (the reference's BLK_H)
+constexpr int kWinW = 8;   // columns per TC block on the LDS path (BLK_W)
+constexpr int kClasses = 5;
+constexpr int kClassWaves[kClasses] = {1, 2, 4, 8, 16};
+constexpr int kClassItems[kClasses] = {8, 16, 16, 16, 16};
+constexpr int kClassCap[kClasses] = {64 * 1 * 8, 64 * 2 * 16, 64 * 4 * 16, 64 * 8 * 16,
+                                     64 * 16 * 16};
+
+// Routing of windows (environment DTC_METCF_PATH): by size, every window
+// through the global path, or every window that fits class c through c.
+constexpr int kPathAuto = 0;
+constexpr int kPathGlobal = 1;
+constexpr int kPathClass0 = 2;
+
 struct MetcfHeader {
   int32_t flags;        // CSR check bits, see kBad*
   int32_t max_col;      // largest column index
   int32_t block_count;  // sum of blocks over non-empty windows
   int32_t rwo_last;     // row_window_offset[W]
-  int32_t colbits;      // bits used for the column in the sort key
-  int32_t sorted_in_1;  // sorted pairs live in buffer 1
-  int32_t pad[58];
+  int32_t colbits;      // bits used for the column in the global-path key
+  int32_t sorted_in_1;  // global-path sorted pairs live in buffer 1
+  int32_t n_big;        // windows on the global path
+  int32_t nnz_big;      // their entries
+  int32_t cls_count[kClasses];  // windows per LDS class
+  int32_t pad[51];
 };
 static_assert(sizeof(MetcfHeader) == 256, "header size");
 
@@ -46,25 +76,50 @@ constexpr int kBadFirst = 1;     // rowptr[0] != 0
 constexpr int kBadLast = 2;      // rowptr[M] != nnz
 constexpr int kBadOrder = 4;     // rowptr decreases
 constexpr int kBadColumn = 8;    // negative column index
+constexpr int kBadClass = 16;    // a window larger than its class (internal)
 
 struct MetcfWs {
   MetcfHeader *hdr;
-  uint64_t *keys0, *keys1;
-  int32_t *vals0, *vals1;
-  int32_t *gscan;    // inclusive unique-column count per sorted element
-  void *tmp;         // scan / sort scratch
+  uint64_t *keys0, *keys1;  // global path: (compacted window, column) keys
+  int32_t *vals0, *vals1;   // global path: CSR edge index
+  int32_t *gscan;           // global path: inclusive unique-column count
+  int32_t *boff;            // staged block data for phase 2 (see WinStage)
+  int32_t *dcol;
+  int32_t *wU;
+  int32_t *lists;           // [kClasses][W] windows of each LDS class
+  int32_t *widx;            // inclusive count of global-path windows up to w
+  int32_t *wcs;             // inclusive count of their entries up to w
+  int32_t *wmap;            // compacted window -> window
+  int32_t *wcls;            // class of each window: -1 empty, LDS class, kClasses global
+  int32_t *bstart;          // global path: first compacted entry of each of its blocks
+  int32_t *cbb;             // global path: first block of each compacted window [n_big + 1]
+  int32_t *csb;             // compacted window -> first compacted entry [n_big + 1]
+  void *tmp;                // scan / sort scratch
   size_t bytes;
 };
 
 MetcfWs carve(void *ws, int64_t nnz, int64_t W) {
   Carver c(ws);
   MetcfWs w;
+  const size_t n = (size_t)std::max<int64_t>(nnz, 1);
+  const size_t nw = (size_t)std::max<int64_t>(W, 1);
   w.hdr = c.take<MetcfHeader>(1);
-  w.keys0 = c.take<uint64_t>((size_t)std::max<int64_t>(nnz, 1));
-  w.keys1 = c.take<uint64_t>((size_t)std::max<int64_t>(nnz, 1));
-  w.vals0 = c.take<int32_t>((size_t)std::max<int64_t>(nnz, 1));
-  w.vals1 = c.take<int32_t>((size_t)std::max<int64_t>(nnz, 1));
-  w.gscan = c.take<int32_t>((size_t)std::max<int64_t>(nnz, 1));
+  w.keys0 = c.take<uint64_t>(n);
+  w.keys1 = c.take<uint64_t>(n);
+  w.vals0 = c.take<int32_t>(n);
+  w.vals1 = c.take<int32_t>(n);
+  w.gscan = c.take<int32_t>(n);
+  w.boff = c.take<int32_t>(n);
+  w.dcol = c.take<int32_t>(n);
+  w.wU = c.take<int32_t>(nw);
+  w.lists = c.take<int32_t>(kClasses * nw);
+  w.widx = c.take<int32_t>(nw);
+  w.wcs = c.take<int32_t>(nw);
+  w.wmap = c.take<int32_t>(nw);
+  w.wcls = c.take<int32_t>(nw);
+  w.bstart = c.take<int32_t>(n);
+  w.cbb = c.take<int32_t>(nw + 1);
+  w.csb = c.take<int32_t>(nw + 1);
   const size_t tmp = std::max({radix_sort_workspace_bytes(nnz), scan_workspace_bytes(nnz),
                                scan_workspace_bytes(W)});
   w.tmp = c.take<char>(tmp);
@@ -78,6 +133,18 @@ int bits_for(int64_t max_value) {
   return b;
 }
 
+int path_mode(int32_t blk_h, int32_t blk_w) {
+  if (blk_h != kWinH || blk_w != kWinW) return kPathGlobal;
+  const char *e = std::getenv("DTC_METCF_PATH");
+  if (!e || !*e || !std::strcmp(e, "auto")) return kPathAuto;
+  if (!std::strcmp(e, "global")) return kPathGlobal;
+  for (int c = 0; c < kClasses; ++c) {
+    const std::string name = "lds" + std::to_string(c);
+    if (name == e) return kPathClass0 + c;
+  }
+  return kPathAuto;
+}
+
 __global__ void k_header_init(MetcfHeader *h) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     h->flags = 0;
@@ -86,6 +153,9 @@ __global__ void k_header_init(MetcfHeader *h) {
     h->rwo_last = 0;
     h->colbits = 0;
     h->sorted_in_1 = 0;
+    h->n_big = 0;
+    h->nnz_big = 0;
+    for (int c = 0; c < kClasses; ++c) h->cls_count[c] = 0;
   }
 }
 
@@ -97,7 +167,8 @@ __global__ void k_header_set(MetcfHeader *h, int colbits, int sorted_in_1) {
 }
 
 // Checks rowptr[0] == 0, rowptr[M] == nnz, rowptr non-decreasing, columns
-// >= 0, and finds the largest column.
+// >= 0, and finds the largest column.  Columns are read 16 bytes per lane
+// when the array is 16-byte aligned.
 __global__ __launch_bounds__(256) void k_validate(const int32_t *rowptr, const int32_t *col,
                                                   int32_t M, int32_t nnz, MetcfHeader *h) {
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -106,11 +177,28 @@ __global__ __launch_bounds__(256) void k_validate(const int32_t *rowptr, const i
   int mx = 0;
   for (int64_t r = tid; r < M; r += stride)
     if (rowptr[r] > rowptr[r + 1]) flags |= kBadOrder;
-  for (int64_t e = tid; e < nnz; e += stride) {
+  const int64_t n4 = ((uintptr_t)col % 16 == 0) ? nnz / 4 : 0;
+  const int4 *col4 = reinterpret_cast<const int4 *>(col);
+  int neg = 0;
+  for (int64_t i = tid; i < n4; i += 4 * stride) {
+    int4 c[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t j = i + u * stride;
+      c[u] = j < n4 ? col4[j] : make_int4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      neg |= c[u].x | c[u].y | c[u].z | c[u].w;
+      mx = max(mx, max(max(c[u].x, c[u].y), max(c[u].z, c[u].w)));
+    }
+  }
+  for (int64_t e = n4 * 4 + tid; e < nnz; e += stride) {
     const int c = col[e];
-    if (c < 0) flags |= kBadColumn;
+    neg |= c;
     mx = max(mx, c);
   }
+  if (neg < 0) flags |= kBadColumn;
   if (tid == 0) {
     if (rowptr[0] != 0) flags |= kBadFirst;
     if (rowptr[M] != nnz) flags |= kBadLast;
@@ -134,27 +222,6 @@ __global__ __launch_bounds__(256) void k_validate(const int32_t *rowptr, const i
   }
 }
 
-// edgeToRow (DTCSpMM_kernel.cu:107-120) and the sort keys, edge-parallel:
-// wavefront w takes the edges [nnz*w/NW, nnz*(w+1)/NW) and walks them 64 at a
-// time.  Lane j holds rowptr[r + 1 + j], the end of row r + j, for the 64 rows
-// from the row r containing the next edge; every edge of a step lies before
-// the last of those ends, and its row is r + (the first j whose end exceeds
-// it), found by a 6-step binary search over the lanes.  A hub row is spread
-// over many wavefronts and short rows share one step, so no wavefront walks a
-// long row serially.  The step's columns are loaded with its row ends.
-__global__ __launch_bounds__(256) void k_rows_keys(const int32_t *rowptr, const int32_t *col,
-                                                   int32_t M, int32_t nnz, int32_t blk_h,
-                                                   int colbits, int32_t *e2r, uint64_t *keys,
-                                                   int32_t *vals) {
-  for_each_edge_row_load(
-      rowptr, M, nnz, [&](int e) { return col[e]; },
-      [&](int idx, int row, int c) {
-        e2r[idx] = row;
-        keys[idx] = ((uint64_t)(row / blk_h) << colbits) | (uint32_t)c;
-        vals[idx] = idx;
-      });
-}
-
 __device__ __forceinline__ int window_begin(const int32_t *rowptr, int w, int blk_h, int M) {
   return rowptr[min((int64_t)w * blk_h, (int64_t)M)];
 }
@@ -162,26 +229,667 @@ __device__ __forceinline__ int window_end(const int32_t *rowptr, int w, int blk_
   return rowptr[min((int64_t)w * blk_h + blk_h, (int64_t)M)];
 }
 
-// blockPartition[w] = ceil(|S_w| / blk_w) for non-empty windows
-// (DTCSpMM_kernel.cu:215-223); the block count sums them.
-__global__ __launch_bounds__(256) void k_window_counts(const int32_t *rowptr, int32_t M,
-                                                       int32_t W, int32_t blk_h,
-                                                       int32_t blk_w, const int32_t *g,
-                                                       int32_t *bp, int32_t *rwo,
-                                                       MetcfHeader *h) {
+// Window classification.  Non-empty windows that fit an LDS class are
+// appended to that class's list (one global atomic per workgroup and class;
+// the order of a list does not affect any output); the others are flagged for
+// the global path (widx = 1, wcs = E), both prefix-summed afterwards.
+__global__ __launch_bounds__(256) void k_classify(const int32_t *rowptr, int32_t M, int32_t W,
+                                                  int32_t blk_h, int mode, int32_t *lists,
+                                                  int32_t *widx, int32_t *wcs, int32_t *wcls,
+                                                  MetcfHeader *h) {
+  __shared__ int s_cnt[kClasses], s_base[kClasses];
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  if (threadIdx.x < kClasses) s_cnt[threadIdx.x] = 0;
+  __syncthreads();
+  int cls = -1;
+  if (w < W) {
+    const int E = window_end(rowptr, w, blk_h, M) - window_begin(rowptr, w, blk_h, M);
+    if (E > 0) {
+      cls = kClasses;
+      if (mode == kPathAuto) {
+        for (int c = kClasses - 1; c >= 0; --c)
+          if (E <= kClassCap[c]) cls = c;
+      } else if (mode >= kPathClass0) {
+        const int c = mode - kPathClass0;
+        if (E <= kClassCap[c]) cls = c;
+      }
+    }
+    widx[w] = cls == kClasses ? 1 : 0;
+    wcs[w] = cls == kClasses ? E : 0;
+    wcls[w] = cls;
+  }
+  uint64_t mask[kClasses];
+  int woff[kClasses];
+#pragma unroll
+  for (int c = 0; c < kClasses; ++c) {
+    mask[c] = __ballot(cls == c);
+    woff[c] = 0;
+    if (mask[c]) {
+      const int leader = __ffsll((unsigned long long)mask[c]) - 1;
+      if (lane == leader) woff[c] = atomicAdd(&s_cnt[c], __popcll(mask[c]));
+      woff[c] = __shfl(woff[c], leader, 64);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < kClasses)
+    s_base[threadIdx.x] = s_cnt[threadIdx.x] ? atomicAdd(&h->cls_count[threadIdx.x], s_cnt[threadIdx.x]) : 0;
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < kClasses; ++c) {
+    if (cls == c) {
+      const int below = (int)__builtin_amdgcn_mbcnt_hi(
+          (uint32_t)(mask[c] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask[c], 0u));
+      lists[(int64_t)c * W + s_base[c] + woff[c] + below] = w;
+    }
+  }
+}
+
+// From the inclusive counts: wmap[w'] = w and csb[w'] = first compacted entry
+// of the w'-th global-path window; the totals go to the header.
+__global__ __launch_bounds__(256) void k_big_map(int32_t W, const int32_t *widx,
+                                                 const int32_t *wcs, int32_t *wmap,
+                                                 int32_t *csb, MetcfHeader *h) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  const int prev = w ? widx[w - 1] : 0;
+  if (widx[w] != prev) {
+    wmap[prev] = w;
+    csb[prev] = w ? wcs[w - 1] : 0;
+  }
+  if (w == W - 1) {
+    h->n_big = widx[W - 1];
+    h->nnz_big = wcs[W - 1];
+    csb[widx[W - 1]] = wcs[W - 1];
+  }
+}
+
+// ------------------------------------------------------------- LDS path
+
+// Block data kept in the workspace by phase 1 (both paths) for phase 2, at
+// CSR positions of the window: block slots s + kb and distinct-column slots
+// s + r stay inside [s, s + E).
+struct WinStage {
+  int32_t *boff;   // first TC-block position of block kb, window-relative
+  int32_t *dcol;   // r-th distinct column of the window
+  int32_t *wU;     // [W] distinct columns of the window
+};
+
+// Largest r in [0, 16) with s_rp[r] <= q: the window row holding local entry
+// q (s_rp[r] = start of row r relative to the window; empty rows share the
+// start of the next row, s_rp[0] = 0 <= q).
+__device__ __forceinline__ int row_of(const int *s_rp, int q) {
+  int r = 0;
+#pragma unroll
+  for (int step = kWinH / 2; step >= 1; step >>= 1)
+    if (s_rp[r + step] <= q) r += step;
+  return r;
+}
+
+// Number of set bits of m in lanes below this one.
+__device__ __forceinline__ int lanes_below(uint64_t m) {
+  return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// s_cnt[v][d] holds the number of entries of digit d among wave v's entries;
+// replaced by the position of wave v's first entry of digit d in the
+// digit-sorted order (digit-major, then wave).  Every thread must call it.
+template <int WAVES>
+__device__ __forceinline__ void digit_offsets(int (*s_cnt)[256], int *s_scan4) {
+  constexpr int NT = 64 * WAVES;
+  constexpr int DPT = NT >= 256 ? 1 : 256 / NT;  // digits per thread
+  constexpr int ACTIVE = 256 / DPT;              // threads holding digits: 64 or 256
+  const int tid = threadIdx.x;
+  int loc[DPT];
+  int sum = 0;
+  if (tid < ACTIVE) {
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+      const int d = tid * DPT + q;
+      int run = 0;
+#pragma unroll
+      for (int vv = 0; vv < WAVES; ++vv) {
+        const int c = s_cnt[vv][d];
+        s_cnt[vv][d] = run;
+        run += c;
+      }
+      loc[q] = sum;
+      sum += run;
+    }
+  }
+  const int inc = wave_inclusive_sum(sum);
+  int excl = inc - sum;
+  if (ACTIVE > 64) {
+    if ((tid & 63) == 63 && tid < ACTIVE) s_scan4[tid >> 6] = inc;
+    __syncthreads();
+    const int wv = tid >> 6;
+#pragma unroll
+    for (int j = 0; j < ACTIVE / 64; ++j) excl += j < wv ? s_scan4[j] : 0;
+  }
+  if (tid < ACTIVE) {
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+      const int d = tid * DPT + q;
+#pragma unroll
+      for (int vv = 0; vv < WAVES; ++vv) s_cnt[vv][d] += excl + loc[q];
+    }
+  }
+  __syncthreads();
+}
+
+// Phase 1 of the LDS path, one workgroup per window of the class list.
+// Local CSR entry q is held by wave q / (64 ITEMS), slot (q / 64) % ITEMS,
+// lane q % 64; after the sort the same layout holds sorted position p.
+//   sort   stable LSD radix sort of (column - window minimum, q) over the
+//          window's column span, 8-bit digits: each wave ranks its entries
+//          slot by slot (one ballot per digit bit gives the lanes of equal
+//          digit, added to per-wave LDS counters), digit_offsets turns the
+//          counts into positions, and key and index move to them through
+//          LDS (one 32-bit + one 16-bit buffer, or for the 16-wave shape one
+//          16-bit buffer in two or three pieces);
+//   ranks  a sorted entry is new when its column differs from its
+//          predecessor's; rank = (new entries up to it) - 1 is its
+//          edgeToColumn, U = number of distinct columns;
+//   blocks block kb holds ranks [8 kb, 8 kb + 8): its entries are the sorted
+//          positions [first of rank 8 kb, first of rank 8 kb + 8); an entry's
+//          TC-block position is the block start plus the number of the
+//          block's entries with a smaller CSR index (the reference's
+//          CSR-order placement inside a block).
+// Writes edgeToRow, edgeToColumn, blockPartition, the tile ids (and values,
+// edge indices) in TC-block order, and the WinStage block data.
+template <int WAVES, int ITEMS>
+__global__ __launch_bounds__(64 * WAVES) void k_window_sort(
+    const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col, int32_t M,
+    const int32_t *__restrict__ list, const float *__restrict__ values, int32_t *__restrict__ e2r,
+    int32_t *__restrict__ e2c, int32_t *__restrict__ bp, uint8_t *__restrict__ tile,
+    float *__restrict__ tcvalues, int32_t *__restrict__ tcedge, WinStage st, MetcfHeader *h) {
+  constexpr int CAP = 64 * WAVES * ITEMS;
+  // key and index move in one step (32-bit + 16-bit buffers) when they fit
+  constexpr bool kWide = CAP * 6 + WAVES * 1024 + CAP / 4 <= 61440;
+  constexpr int kRaw = kWide ? CAP + CAP / 2 : CAP / 2;
+  __shared__ uint32_t s_raw[kRaw];
+  __shared__ int s_cnt[WAVES][256];
+  __shared__ uint16_t s_bs[CAP / kWinW + 1];
+  __shared__ int s_rp[kWinH + 1];
+  __shared__ int s_min[WAVES], s_max[WAVES], s_wtot[WAVES];
+  __shared__ uint32_t s_last[WAVES];
+  __shared__ int s_scan4[4];
+  uint32_t *s_k = s_raw;                                                 // kWide only
+  uint16_t *s_x = reinterpret_cast<uint16_t *>(kWide ? s_raw + CAP : s_raw);
+  const int tid = threadIdx.x, v = tid >> 6, lane = tid & 63;
+  const int vbase = v * 64 * ITEMS;
+  const int w = list[blockIdx.x];
+  const int s = window_begin(rowptr, w, kWinH, M);
+  const int E = window_end(rowptr, w, kWinH, M) - s;
+  if (E <= 0 || E > CAP) {  // the class list guarantees 0 < E <= CAP
+    if (tid == 0) atomicOr(&h->flags, kBadClass);
+    return;
+  }
+  if (tid <= kWinH) s_rp[tid] = rowptr[min((int64_t)w * kWinH + tid, (int64_t)M)] - s;
+  __syncthreads();
+
+  // key[k]: column - window minimum.  ix[k]: local index (low 16 bits)
+  // and, during a pass, the entry's destination, after the sort its running
+  // count of new columns in the wave (high 16 bits).
+  uint32_t key[ITEMS];
+  uint32_t ix[ITEMS];
+  int cmin = INT32_MAX, cmax = 0;
+  // Branch-free loads (clamped to the window) so that all of them are in
+  // flight together.
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    const int q = vbase + k * 64 + lane;
+    ix[k] = (uint32_t)q;
+    const int c = col[s + min(q, E - 1)];
+    key[k] = q < E ? (uint32_t)c : 0u;
+    cmin = q < E ? min(cmin, c) : cmin;
+    cmax = q < E ? max(cmax, c) : cmax;
+  }
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    const int q = vbase + k * 64 + lane;
+    if (q < E) e2r[s + q] = w * kWinH + row_of(s_rp, q);
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    cmin = min(cmin, __shfl_xor(cmin, d, 64));
+    cmax = max(cmax, __shfl_xor(cmax, d, 64));
+  }
+  if (lane == 0) {
+    s_min[v] = cmin;
+    s_max[v] = cmax;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int vv = 0; vv < WAVES; ++vv) {
+    cmin = min(cmin, s_min[vv]);
+    cmax = max(cmax, s_max[vv]);
+  }
+  const uint32_t span = (uint32_t)(cmax - cmin);
+  const int bits = span ? 32 - __clz(span) : 0;
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k)
+    if (vbase + k * 64 + lane < E) key[k] -= (uint32_t)cmin;
+
+  for (int shift = 0; shift < bits; shift += 8) {
+    const int nbits = min(8, bits - shift);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) s_cnt[v][j * 64 + lane] = 0;
+    wave_lds_sync();
+    if (shift == 0) {
+      // The first pass need not be stable (there is no earlier order to
+      // keep; entries of equal column may end in any order, which no output
+      // depends on): rank by LDS atomics on the wave's counters.
+#pragma unroll
+      for (int k = 0; k < ITEMS; ++k) {
+        if (vbase + k * 64 < E) {  // wave-uniform
+          const bool valid = vbase + k * 64 + lane < E;
+          const int d = (int)(key[k] & 255u);
+          const int r = valid ? atomicAdd(&s_cnt[v][d], 1) : 0;
+          ix[k] = (ix[k] & 0xffffu) | ((uint32_t)r << 16);
+        }
+      }
+    } else {
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      if (vbase + k * 64 < E) {  // wave-uniform
+        const bool valid = vbase + k * 64 + lane < E;
+        const int d = (int)((key[k] >> shift) & 255u);
+        uint64_t peers = __ballot(valid);
+        for (int bit = 0; bit < nbits; ++bit) {
+          const bool b = (d >> bit) & 1;
+          const uint64_t m = __ballot(b);
+          peers &= b ? m : ~m;
+        }
+        const int r = lanes_below(peers);
+        const int c = valid ? s_cnt[v][d] : 0;
+        ix[k] = (ix[k] & 0xffffu) | ((uint32_t)(c + r) << 16);
+        wave_lds_sync();
+        if (valid && r == 0) s_cnt[v][d] = c + __popcll(peers);
+        wave_lds_sync();
+      }
+    }
+    }
+    __syncthreads();
+    digit_offsets<WAVES>(s_cnt, s_scan4);
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k)
+      if (vbase + k * 64 + lane < E) ix[k] += (uint32_t)s_cnt[v][(key[k] >> shift) & 255u] << 16;
+    if (kWide) {
+#pragma unroll
+      for (int k = 0; k < ITEMS; ++k) {
+        if (vbase + k * 64 + lane < E) {
+          s_k[ix[k] >> 16] = key[k];
+          s_x[ix[k] >> 16] = (uint16_t)ix[k];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < ITEMS; ++k) {
+        const int q = vbase + k * 64 + lane;
+        if (vbase + k * 64 < E && q < E) {
+          key[k] = s_k[q];
+          ix[k] = s_x[q];
+        }
+      }
+      __syncthreads();
+    } else {
+      // through the 16-bit buffer: the high key half (if the span needs it),
+      // the low key half, then the local index
+      uint32_t hi[ITEMS];
+#pragma unroll
+      for (int k = 0; k < ITEMS; ++k) hi[k] = 0;
+      if (bits > 16) {
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k)
+          if (vbase + k * 64 + lane < E) s_x[ix[k] >> 16] = (uint16_t)(key[k] >> 16);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+          const int q = vbase + k * 64 + lane;
+          if (vbase + k * 64 < E && q < E) hi[k] = s_x[q];
+        }
+        __syncthreads();
+      }
+#pragma unroll
+      for (int k = 0; k < ITEMS; ++k)
+        if (vbase + k * 64 + lane < E) s_x[ix[k] >> 16] = (uint16_t)key[k];
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < ITEMS; ++k) {
+        const int q = vbase + k * 64 + lane;
+        if (vbase + k * 64 < E && q < E) key[k] = (hi[k] << 16) | s_x[q];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < ITEMS; ++k)
+        if (vbase + k * 64 + lane < E) s_x[ix[k] >> 16] = (uint16_t)ix[k];
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < ITEMS; ++k) {
+        const int q = vbase + k * 64 + lane;
+        if (vbase + k * 64 < E && q < E) ix[k] = s_x[q];
+      }
+      __syncthreads();
+    }
+  }
+  // s_x[p] = local index of sorted position p (left there by the last move).
+  if (bits == 0) {
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k)
+      if (vbase + k * 64 + lane < E) s_x[vbase + k * 64 + lane] = (uint16_t)ix[k];
+  }
+
+  // ranks of the distinct columns: high half of ix = new columns of the wave
+  // up to and including the entry
+  if (lane == 63) s_last[v] = key[ITEMS - 1];
+  __syncthreads();
+  const uint32_t prev_wave = v > 0 ? s_last[v - 1] : 0u;
+  uint32_t newmask = 0;
+  int run = 0;
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    if (vbase + k * 64 < E) {  // wave-uniform
+      const int q = vbase + k * 64 + lane;
+      const uint32_t up = __shfl_up(key[k], 1, 64);
+      const uint32_t from_prev =
+          k > 0 ? (uint32_t)__builtin_amdgcn_readlane((int)key[k > 0 ? k - 1 : 0], 63) : prev_wave;
+      const uint32_t pk = lane == 0 ? from_prev : up;
+      const bool isnew = q < E && (q == 0 || key[k] != pk);
+      const uint64_t m = __ballot(isnew);
+      const int rk = run + lanes_below(m) + (isnew ? 1 : 0);
+      run += __popcll(m);
+      ix[k] = (ix[k] & 0xffffu) | ((uint32_t)rk << 16);
+      newmask |= isnew ? (1u << k) : 0u;
+    }
+  }
+  if (lane == 0) s_wtot[v] = run;
+  __syncthreads();
+  int pre = 0, U = 0;
+#pragma unroll
+  for (int vv = 0; vv < WAVES; ++vv) {
+    const int t = s_wtot[vv];
+    pre += vv < v ? t : 0;
+    U += t;
+  }
+  const int nb = (U + kWinW - 1) / kWinW;
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    if ((newmask >> k) & 1u) {
+      const int q = vbase + k * 64 + lane;
+      const int rank = (int)(ix[k] >> 16) + pre - 1;
+      st.dcol[s + rank] = (int32_t)(key[k] + (uint32_t)cmin);
+      if (rank % kWinW == 0) {
+        s_bs[rank / kWinW] = (uint16_t)q;
+        st.boff[s + rank / kWinW] = q;
+      }
+    }
+  }
+  if (tid == 0) {
+    s_bs[nb] = (uint16_t)E;
+    bp[w] = nb;
+    st.wU[w] = U;
+  }
+  __syncthreads();
+  // TC-block position of every entry; its tile id, value and edge index
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    const int q = vbase + k * 64 + lane;
+    if (vbase + k * 64 < E && q < E) {
+      const int rank = (int)(ix[k] >> 16) + pre - 1;
+      const uint32_t i = ix[k] & 0xffffu;
+      const int kb = rank / kWinW, c = rank % kWinW;
+      const int lo = s_bs[kb], hi = s_bs[kb + 1];
+      int pos = lo;
+      for (int j = lo; j < hi; ++j) pos += (uint32_t)s_x[j] < i ? 1 : 0;
+      if (pos < E) {
+        tile[s + pos] = (uint8_t)(row_of(s_rp, (int)i) * kWinW + c);
+        if (tcvalues) tcvalues[s + pos] = values ? values[s + i] : 1.0f;
+        if (tcedge) tcedge[s + pos] = s + (int)i;
+      }
+    }
+  }
+  __syncthreads();
+  // edgeToColumn in CSR order through LDS
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k)
+    if (vbase + k * 64 + lane < E) s_x[ix[k] & 0xffffu] = (uint16_t)((int)(ix[k] >> 16) + pre - 1);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    const int q = vbase + k * 64 + lane;
+    if (vbase + k * 64 < E && q < E) e2c[s + q] = (int)s_x[q];
+  }
+}
+
+// Phase 2 (every window, both paths): a workgroup takes kOutBlocks
+// consecutive TC blocks.  Two waves find the windows of its first and last
+// block (rwo[w] <= b < rwo[w+1]) by cooperative searches; the row-window
+// offsets in between go to LDS (when they fit), where each block finds its
+// window.  Each block's offset and row id, and then SparseAToX (the window's
+// distinct columns from kb * blk_w on, zero past U, consecutive threads on
+// consecutive entries) come from the WinStage data.
+constexpr int kOutBlocks = 512;
+constexpr int kOutWin = 1024;  // row-window offsets kept in LDS
+__global__ __launch_bounds__(256) void k_blocks_out(const int32_t *__restrict__ rowptr, int32_t M,
+                                                    int32_t W, int32_t blk_h, int32_t blk_w,
+                                                    int32_t T, WinStage st,
+                                                    const int32_t *__restrict__ rwo,
+                                                    int32_t *__restrict__ off,
+                                                    int32_t *__restrict__ rowid,
+                                                    int32_t *__restrict__ atox) {
+  __shared__ int s_rwo[kOutWin + 1];
+  __shared__ int s_ends[2];
+  __shared__ int s_base[kOutBlocks], s_kb[kOutBlocks], s_U[kOutBlocks];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int first = blockIdx.x * kOutBlocks;
+  if (first >= T) return;  // workgroup-uniform
+  const int nblk = min(kOutBlocks, T - first);
+  if (wave < 2) {
+    const int b = wave ? first + nblk - 1 : first;
+    // window of block b: the last w with rwo[w] <= b
+    const int w = wave_lower_bound(rwo, 0, W + 1, b + 1) - 1;
+    if (lane == 0) s_ends[wave] = w;
+  }
+  __syncthreads();
+  const int wa = s_ends[0], wb = s_ends[1];
+  const bool in_lds = wb - wa + 1 <= kOutWin;
+  if (in_lds)
+    for (int k = tid; k <= wb - wa + 1; k += blockDim.x) s_rwo[k] = rwo[wa + k];
+  __syncthreads();
+  for (int t = tid; t < nblk; t += blockDim.x) {
+    const int b = first + t;
+    int w;
+    if (in_lds) {
+      int lo = 1, hi = wb - wa + 1;  // first k in [1, wb - wa + 1] with s_rwo[k] > b
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (s_rwo[mid] <= b) lo = mid + 1;
+        else hi = mid;
+      }
+      w = wa + lo - 1;
+    } else {
+      w = lower_bound_dev(rwo, wa + 1, wb + 1, b + 1) - 1;
+    }
+    const int s = rowptr[min((int64_t)w * blk_h, (int64_t)M)];
+    const int kb = b - (in_lds ? s_rwo[w - wa] : rwo[w]);
+    off[b] = s + st.boff[s + kb];
+    rowid[b] = w;
+    s_base[t] = s + kb * blk_w;
+    s_kb[t] = kb * blk_w;
+    s_U[t] = st.wU[w];
+  }
+  __syncthreads();
+  const int n_atox = nblk * blk_w;
+  const int64_t a0 = (int64_t)first * blk_w;
+  for (int e = tid; e < n_atox; e += blockDim.x) {
+    const int t = e / blk_w, c = e - t * blk_w;
+    atox[a0 + e] = s_kb[t] + c < s_U[t] ? st.dcol[s_base[t] + c] : 0;
+  }
+}
+
+// ---------------------------------------------------------- global path
+
+// Keys of the global-path windows: compacted entry j of window w'
+// (entries [csb[w'], csb[w'+1])) is CSR entry rowptr[w*blk_h] + j - csb[w']
+// of window w = wmap[w']; key (w', column), value = CSR edge index.  Also
+// writes edgeToRow of those entries.  A workgroup takes 4,096 consecutive
+// compacted entries; two of its threads find the windows of the first and
+// the last one, and each entry searches only between those two.
+constexpr int kBigChunk = 4096;
+__global__ __launch_bounds__(256) void k_big_keys(const int32_t *rowptr, const int32_t *col,
+                                                  int32_t M, int32_t blk_h, int colbits,
+                                                  const MetcfHeader *h, const int32_t *wmap,
+                                                  const int32_t *csb, int32_t *e2r,
+                                                  uint64_t *keys, int32_t *vals) {
+  __shared__ int s_w[2];
+  const int n_big = h->n_big, nnz_big = h->nnz_big;
+  const int64_t first = (int64_t)blockIdx.x * kBigChunk;
+  if (first >= nnz_big) return;  // workgroup-uniform
+  const int last = (int)min<int64_t>(first + kBigChunk - 1, nnz_big - 1);
+  // window of entry j: the last w' with csb[w'] <= j
+  if (threadIdx.x < 2)
+    s_w[threadIdx.x] = lower_bound_dev(csb, 0, n_big + 1, (threadIdx.x ? last : (int)first) + 1) - 1;
+  __syncthreads();
+  const int wa = s_w[0], wb = s_w[1];
+  for (int j = (int)first + threadIdx.x; j <= last; j += blockDim.x) {
+    const int wp = wa == wb ? wa : lower_bound_dev(csb, wa, wb + 1, j + 1) - 1;
+    const int w = wmap[wp];
+    const int64_t r0 = (int64_t)w * blk_h;
+    const int s = rowptr[r0];
+    const int e = s + (j - csb[wp]);
+    // row: the last r in [r0, min(r0 + blk_h, M)) with rowptr[r] <= e
+    const int rend = (int)min<int64_t>(r0 + blk_h, M);
+    const int row = lower_bound_dev(rowptr, (int)r0, rend, e + 1) - 1;
+    e2r[e] = row;
+    keys[j] = ((uint64_t)wp << colbits) | (uint32_t)col[e];
+    vals[j] = e;
+  }
+}
+
+struct BigView {
+  const uint64_t *keys;
+  const int32_t *vals;
+  const int32_t *g;
+  const int32_t *csb;
+  const int32_t *wmap;
+  int colbits;
+  __device__ __forceinline__ int window(int j) const { return (int)(keys[j] >> colbits); }
+  // unique-column count before compacted window wp
+  __device__ __forceinline__ int base(int wp) const {
+    const int c = csb[wp];
+    return c > 0 ? g[c - 1] : 0;
+  }
+};
+
+__device__ __forceinline__ BigView big_view(const MetcfHeader *h, const uint64_t *keys0,
+                                            const uint64_t *keys1, const int32_t *vals0,
+                                            const int32_t *vals1, const int32_t *g,
+                                            const int32_t *csb, const int32_t *wmap) {
+  BigView bv;
+  bv.keys = h->sorted_in_1 ? keys1 : keys0;
+  bv.vals = h->sorted_in_1 ? vals1 : vals0;
+  bv.g = g;
+  bv.csb = csb;
+  bv.wmap = wmap;
+  bv.colbits = h->colbits;
+  return bv;
+}
+
+// blockPartition of the global-path windows: ceil(|S_w| / blk_w)
+// (DTCSpMM_kernel.cu:215-223); also |S_w| (WinStage) and the block count per
+// compacted window (prefix-summed into cbb).
+__global__ __launch_bounds__(256) void k_big_counts(const MetcfHeader *h, const int32_t *g,
+                                                    const int32_t *csb, const int32_t *wmap,
+                                                    int32_t blk_w, int32_t *bp, int32_t *wU,
+                                                    int32_t *nbc) {
+  const int wp = blockIdx.x * blockDim.x + threadIdx.x;
+  if (wp >= h->n_big) return;
+  const int c0 = csb[wp], c1 = csb[wp + 1];
+  const int U = g[c1 - 1] - (c0 > 0 ? g[c0 - 1] : 0);
+  const int nb = (U - 1 + blk_w) / blk_w;
+  bp[wmap[wp]] = nb;
+  wU[wmap[wp]] = U;
+  nbc[wp] = nb;
+}
+
+// Per sorted entry j of the global path: edgeToColumn[e] = rank of col(e) in
+// its window (DTCSpMM_kernel.cu:224-227); the first entry of each distinct
+// column stores the column (WinStage dcol), the first of each block its
+// window-relative position (WinStage boff) and its compacted position
+// (bstart, for the tile pass).
+__global__ __launch_bounds__(256) void k_big_stage(
+    const int32_t *rowptr, int32_t M, int32_t blk_h, int32_t blk_w, const MetcfHeader *h,
+    const uint64_t *keys0, const uint64_t *keys1, const int32_t *vals0, const int32_t *vals1,
+    const int32_t *g, const int32_t *csb, const int32_t *wmap, const int32_t *cbb,
+    int32_t *e2c, WinStage st, int32_t *bstart) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= h->nnz_big) return;
+  const BigView bv = big_view(h, keys0, keys1, vals0, vals1, g, csb, wmap);
+  const int wp = bv.window(j);
+  const int r = g[j] - bv.base(wp) - 1;
+  e2c[bv.vals[j]] = r;
+  if (j > 0 && g[j - 1] == g[j]) return;
+  const int w = wmap[wp];
+  const int s = window_begin(rowptr, w, blk_h, M);
+  st.dcol[s + r] = (int)(bv.keys[j] & ((1ull << bv.colbits) - 1));
+  if (r % blk_w == 0) {
+    const int kb = r / blk_w;
+    st.boff[s + kb] = j - csb[wp];
+    bstart[cbb[wp] + kb] = j;
+  }
+}
+
+// Tile ids (and the optional values / edge indices) of the global-path
+// windows in TC-block order.  A block's entries are one contiguous range of
+// the sorted order (its columns are consecutive condensed columns of one
+// window), [bstart of the block, bstart of the next or the window end); an
+// entry's slot is the block start plus the number of the block's entries
+// with a smaller CSR index, one pass over that range (<= blk_h * blk_w
+// entries).
+__global__ __launch_bounds__(256) void k_big_tile_ids(
+    const int32_t *rowptr, const int32_t *e2r, int32_t M, int32_t blk_h, int32_t blk_w,
+    const MetcfHeader *h, const uint64_t *keys0, const uint64_t *keys1, const int32_t *vals0,
+    const int32_t *vals1, const int32_t *g, const int32_t *csb, const int32_t *wmap,
+    const int32_t *cbb, const int32_t *bstart, const float *values, uint8_t *tile,
+    float *tcvalues, int32_t *tcedge) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= h->nnz_big) return;
+  const BigView bv = big_view(h, keys0, keys1, vals0, vals1, g, csb, wmap);
+  const int wp = bv.window(j);
+  const int w = wmap[wp];
+  const int c0 = csb[wp], c1 = csb[wp + 1];
+  const int base = bv.base(wp);
+  const int U = g[c1 - 1] - base;
+  const int r = g[j] - base - 1;
+  const int kb = r / blk_w;
+  const int c = r % blk_w;
+  const int delta = window_begin(rowptr, w, blk_h, M) - c0;  // CSR position - compacted position
+  const int lo = bstart[cbb[wp] + kb];
+  const int hi = (kb + 1) * blk_w >= U ? c1 : bstart[cbb[wp] + kb + 1];
+  const int myv = bv.vals[j];
+  int pos = lo;
+#pragma unroll 8
+  for (int t = lo; t < hi; ++t) pos += bv.vals[t] < myv;
+  if (pos < c0 || pos >= c1) return;  // unreachable for a consistent build; keeps writes in bounds
+  const int dst = pos + delta;
+  const unsigned row_local = (unsigned)e2r[myv] % (unsigned)blk_h;
+  tile[dst] = (uint8_t)(row_local * (unsigned)blk_w + (unsigned)c);
+  if (tcvalues) tcvalues[dst] = values ? values[myv] : 1.0f;
+  if (tcedge) tcedge[dst] = myv;
+}
+
+// Sum of blockPartition over non-empty windows (the row-window offsets must
+// end at it; a non-zero entry of an empty window means the caller did not
+// zero-fill blockPartition).
+__global__ __launch_bounds__(256) void k_sum_blocks(const int32_t *rowptr, int32_t M, int32_t W,
+                                                    int32_t blk_h, const int32_t *bp,
+                                                    int32_t *rwo, MetcfHeader *h) {
   __shared__ int smem[4];
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   int nb = 0;
-  if (w < W) {
-    const int s = window_begin(rowptr, w, blk_h, M);
-    const int e = window_end(rowptr, w, blk_h, M);
-    if (e > s) {
-      const int base = s > 0 ? g[s - 1] : 0;
-      const int U = g[e - 1] - base;
-      nb = (U - 1 + blk_w) / blk_w;
-      bp[w] = nb;
-    }
-  }
+  if (w < W && window_end(rowptr, w, blk_h, M) > window_begin(rowptr, w, blk_h, M)) nb = bp[w];
   int total;
   block256_inclusive_sum(nb, smem, total);
   if (threadIdx.x == 0 && total) atomicAdd(&h->block_count, total);
@@ -190,95 +898,6 @@ __global__ __launch_bounds__(256) void k_window_counts(const int32_t *rowptr, in
 
 __global__ void k_copy_last(const int32_t *rwo, int32_t W, MetcfHeader *h) {
   if (threadIdx.x == 0 && blockIdx.x == 0) h->rwo_last = rwo[W];
-}
-
-// edgeToColumn[e] = rank of col(e) in its window (DTCSpMM_kernel.cu:224-227).
-__global__ __launch_bounds__(256) void k_edge_to_column(const int32_t *rowptr, int32_t M,
-                                                        int32_t nnz, int32_t blk_h,
-                                                        const MetcfHeader *h,
-                                                        const uint64_t *keys0,
-                                                        const uint64_t *keys1,
-                                                        const int32_t *vals0,
-                                                        const int32_t *vals1,
-                                                        const int32_t *g, int32_t *e2c) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nnz) return;
-  const int colbits = h->colbits;
-  const uint64_t *keys = h->sorted_in_1 ? keys1 : keys0;
-  const int32_t *vals = h->sorted_in_1 ? vals1 : vals0;
-  const int w = (int)(keys[i] >> colbits);
-  const int s = window_begin(rowptr, w, blk_h, M);
-  const int base = s > 0 ? g[s - 1] : 0;
-  e2c[vals[i]] = g[i] - base - 1;
-}
-
-// Phase 2a: SparseAToX, block offsets, block row ids, unique positions.
-__global__ __launch_bounds__(256) void k_fill_blocks(
-    const int32_t *rowptr, const int32_t *rwo, int32_t M, int32_t nnz, int32_t blk_h,
-    int32_t blk_w, int32_t T, const MetcfHeader *h, const uint64_t *keys0,
-    const uint64_t *keys1, const int32_t *g, int32_t *off, int32_t *rowid, int32_t *atox) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nnz) return;
-  const int colbits = h->colbits;
-  const uint64_t *keys = h->sorted_in_1 ? keys1 : keys0;
-  const uint64_t key = keys[i];
-  const int gi = g[i];
-  const bool is_new = (i == 0) || (g[i - 1] != gi);
-  if (i == 0) off[T] = nnz;
-  if (!is_new) return;
-  const int w = (int)(key >> colbits);
-  const int colv = (int)(key & ((1ull << colbits) - 1));
-  const int s = window_begin(rowptr, w, blk_h, M);
-  const int e = window_end(rowptr, w, blk_h, M);
-  const int base = s > 0 ? g[s - 1] : 0;
-  const int r = gi - base - 1;
-  const int64_t b = (int64_t)rwo[w] + r / blk_w;
-  const int c = r % blk_w;
-  atox[b * blk_w + c] = colv;
-  if (c == 0) {
-    off[b] = i;
-    rowid[b] = w;
-  }
-  const int U = g[e - 1] - base;
-  if (r == U - 1)
-    for (int cc = c + 1; cc < blk_w; ++cc) atox[b * blk_w + cc] = 0;
-}
-
-// Phase 2b: tile ids (and the optional reordered values / edge ids).  Within
-// a block the reference stores elements in CSR order; here an element's slot
-// is its block offset plus the number of elements of the same block with a
-// smaller CSR index.  A block's elements are one contiguous range of the
-// sorted order, [off[b], off[b+1]) (its columns are consecutive condensed
-// columns of one window), so that count is one pass over the range: at most
-// blk_h * blk_w = 128 loads that the lanes of one block share (the same
-// addresses), from lines the neighbouring lanes have just brought in.
-__global__ __launch_bounds__(256) void k_tile_ids(
-    const int32_t *rowptr, const int32_t *rwo, const int32_t *e2r, int32_t M, int32_t nnz,
-    int32_t blk_h, int32_t blk_w, const MetcfHeader *h, const uint64_t *keys0,
-    const uint64_t *keys1, const int32_t *vals0, const int32_t *vals1, const int32_t *g,
-    const int32_t *off, const float *values, uint8_t *tile, float *tcvalues, int32_t *tcedge) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nnz) return;
-  const int colbits = h->colbits;
-  const uint64_t *keys = h->sorted_in_1 ? keys1 : keys0;
-  const int32_t *vals = h->sorted_in_1 ? vals1 : vals0;
-  const int w = (int)(keys[i] >> colbits);
-  const int s = window_begin(rowptr, w, blk_h, M);
-  const int base = s > 0 ? g[s - 1] : 0;
-  const int r = g[i] - base - 1;
-  const int kb = r / blk_w;
-  const int c = r % blk_w;
-  const int64_t b = (int64_t)rwo[w] + kb;
-  const int myv = vals[i];
-  const int lo = off[b], hi = off[b + 1];
-  int pos = lo;
-#pragma unroll 8
-  for (int j = lo; j < hi; ++j) pos += vals[j] < myv;
-  if (pos >= hi) return;  // unreachable for a consistent build; keeps writes in bounds
-  const unsigned row_local = (unsigned)e2r[myv] % (unsigned)blk_h;
-  tile[pos] = (uint8_t)(row_local * (unsigned)blk_w + (unsigned)c);
-  if (tcvalues) tcvalues[pos] = values ? values[myv] : 1.0f;
-  if (tcedge) tcedge[pos] = myv;
 }
 
 __global__ void k_set_i32(int32_t *p, int32_t v) {
@@ -296,6 +915,27 @@ dtc_status_t check_shape(int32_t num_nodes, int32_t nnz, int32_t blk_h, int32_t 
     return fail(DTC_ERROR_INVALID_VALUE, "blockSize_h and blockSize_w must be >= 1");
   if (blk_h > 4096 || blk_w > 4096)
     return fail(DTC_ERROR_UNSUPPORTED, "block sizes above 4096 are not supported");
+  return DTC_SUCCESS;
+}
+
+WinStage win_stage(const MetcfWs &ws) {
+  WinStage st;
+  st.boff = ws.boff;
+  st.dcol = ws.dcol;
+  st.wU = ws.wU;
+  return st;
+}
+
+template <int C>
+dtc_status_t launch_window_sort(const int32_t *rowptr, const int32_t *col, int32_t M, int count,
+                                const MetcfWs &ws, int64_t W, const float *values, int32_t *e2r,
+                                int32_t *e2c, int32_t *bp, uint8_t *tile, float *tcvalues,
+                                int32_t *tcedge, hipStream_t stream) {
+  if (count <= 0) return DTC_SUCCESS;
+  k_window_sort<kClassWaves[C], kClassItems[C]><<<(unsigned)count, 64 * kClassWaves[C], 0, stream>>>(
+      rowptr, col, M, ws.lists + (int64_t)C * W, values, e2r, e2c, bp, tile, tcvalues, tcedge,
+      win_stage(ws), ws.hdr);
+  DTC_LAUNCH_CHECK("window_sort");
   return DTC_SUCCESS;
 }
 
@@ -319,9 +959,11 @@ dtc_status_t dtc_metcf_workspace_size(int32_t num_nodes, int32_t nnz, int32_t bl
 
 dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *node_pointer,
                                      int32_t num_nodes, int32_t nnz, int32_t blk_h,
-                                     int32_t blk_w, int32_t *block_partition,
-                                     int32_t *edge_to_column, int32_t *edge_to_row,
-                                     int32_t *row_window_offset, int32_t *block_count,
+                                     int32_t blk_w, const float *values,
+                                     int32_t *block_partition, int32_t *edge_to_column,
+                                     int32_t *edge_to_row, int32_t *row_window_offset,
+                                     uint8_t *tcblocktile_id, float *tcblock_values,
+                                     int32_t *tcblock_edge, int32_t *block_count,
                                      void *workspace, size_t workspace_bytes,
                                      dtc_stream_t stream_) {
   clear_error();
@@ -330,25 +972,41 @@ dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *no
   if (st != DTC_SUCCESS) return st;
   if (!node_pointer || !row_window_offset || !block_count)
     return fail(DTC_ERROR_INVALID_VALUE, "node_pointer, row_window_offset and block_count are required");
-  if (nnz > 0 && (!edge_list || !edge_to_column || !edge_to_row))
-    return fail(DTC_ERROR_INVALID_VALUE, "edge arrays are required when nnz > 0");
+  if (nnz > 0 && (!edge_list || !edge_to_column || !edge_to_row || !tcblocktile_id))
+    return fail(DTC_ERROR_INVALID_VALUE, "edge arrays and tcblocktile_id are required when nnz > 0");
   const int64_t W = ceil_div(num_nodes, blk_h);
   if (W > 0 && !block_partition) return fail(DTC_ERROR_INVALID_VALUE, "block_partition is NULL");
   MetcfWs ws = carve(workspace, nnz, W);
   if (!workspace || workspace_bytes < ws.bytes)
     return fail(DTC_ERROR_WORKSPACE, "ME-TCF workspace needs " + std::to_string(ws.bytes) + " bytes");
+  const int mode = path_mode(blk_h, blk_w);
 
-  // 1. CSR check and largest column (first host synchronisation).
+  // 1. CSR check, largest column and the window classes (first host
+  //    synchronisation).
   k_header_init<<<1, 64, 0, stream>>>(ws.hdr);
   DTC_LAUNCH_CHECK("header_init");
   {
+    // few workgroups: each adds one atomic on the same header words
     const int64_t work = std::max<int64_t>(num_nodes, nnz);
-    const unsigned grid = (unsigned)std::min<int64_t>(4096, grid_for(work, 256));
+    const unsigned grid = (unsigned)std::min<int64_t>(1024, grid_for(work, 256));
     k_validate<<<grid, 256, 0, stream>>>(node_pointer, edge_list, num_nodes, nnz, ws.hdr);
     DTC_LAUNCH_CHECK("validate");
   }
+  if (W > 0) {
+    k_classify<<<grid_for(W, 256), 256, 0, stream>>>(node_pointer, num_nodes, (int32_t)W, blk_h,
+                                                      mode, ws.lists, ws.widx, ws.wcs, ws.wcls,
+                                                      ws.hdr);
+    DTC_LAUNCH_CHECK("classify");
+    st = scan_i32(ws.widx, ws.widx, W, true, ws.tmp, stream);
+    if (st != DTC_SUCCESS) return st;
+    st = scan_i32(ws.wcs, ws.wcs, W, true, ws.tmp, stream);
+    if (st != DTC_SUCCESS) return st;
+    k_big_map<<<grid_for(W, 256), 256, 0, stream>>>((int32_t)W, ws.widx, ws.wcs, ws.wmap, ws.csb,
+                                                     ws.hdr);
+    DTC_LAUNCH_CHECK("big_map");
+  }
   MetcfHeader host{};
-  DTC_HIP_CHECK(hipMemcpyAsync(&host, ws.hdr, 16, hipMemcpyDeviceToHost, stream));
+  DTC_HIP_CHECK(hipMemcpyAsync(&host, ws.hdr, 64, hipMemcpyDeviceToHost, stream));
   DTC_HIP_CHECK(hipStreamSynchronize(stream));
   if (host.flags) {
     std::string why;
@@ -365,44 +1023,73 @@ dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *no
     return DTC_SUCCESS;
   }
   const int colbits = std::max(1, bits_for(host.max_col));
-  const int wbits = bits_for(W - 1);
-  const int key_bits = colbits + wbits;
 
-  bool in_1 = false;
-  if (nnz > 0) {
-    // 2. Keys (window, column) with the CSR edge index as value; stable sort.
-    const unsigned grid = (unsigned)std::min<int64_t>(65536, grid_for(nnz, 4096));
-    k_rows_keys<<<grid, 256, 0, stream>>>(node_pointer, edge_list, num_nodes, nnz, blk_h, colbits,
-                                          edge_to_row, ws.keys0, ws.vals0);
-    DTC_LAUNCH_CHECK("rows_keys");
-    st = radix_sort_pairs(ws.keys0, ws.vals0, ws.keys1, ws.vals1, nnz, key_bits, &in_1, ws.tmp,
+  // 2. LDS path, largest class first: every output of the window sized by
+  //    its entries, and its block data for phase 2.
+  for (int c = kClasses - 1; c >= 0 && st == DTC_SUCCESS; --c) {
+    const int n = host.cls_count[c];
+    switch (c) {
+      case 4: st = launch_window_sort<4>(node_pointer, edge_list, num_nodes, n, ws, W, values, edge_to_row, edge_to_column, block_partition, tcblocktile_id, tcblock_values, tcblock_edge, stream); break;
+      case 3: st = launch_window_sort<3>(node_pointer, edge_list, num_nodes, n, ws, W, values, edge_to_row, edge_to_column, block_partition, tcblocktile_id, tcblock_values, tcblock_edge, stream); break;
+      case 2: st = launch_window_sort<2>(node_pointer, edge_list, num_nodes, n, ws, W, values, edge_to_row, edge_to_column, block_partition, tcblocktile_id, tcblock_values, tcblock_edge, stream); break;
+      case 1: st = launch_window_sort<1>(node_pointer, edge_list, num_nodes, n, ws, W, values, edge_to_row, edge_to_column, block_partition, tcblocktile_id, tcblock_values, tcblock_edge, stream); break;
+      default: st = launch_window_sort<0>(node_pointer, edge_list, num_nodes, n, ws, W, values, edge_to_row, edge_to_column, block_partition, tcblocktile_id, tcblock_values, tcblock_edge, stream); break;
+    }
+  }
+  if (st != DTC_SUCCESS) return st;
+
+  // 3. Global path: keys (compacted window, column), value = CSR edge index;
+  //    stable sort; unique-column numbering; blockPartition, edgeToColumn,
+  //    block data, tile ids.
+  if (host.nnz_big > 0) {
+    const int64_t nb = host.nnz_big;
+    k_big_keys<<<grid_for(nb, kBigChunk), 256, 0, stream>>>(node_pointer, edge_list, num_nodes,
+                                                            blk_h, colbits, ws.hdr, ws.wmap,
+                                                            ws.csb, edge_to_row, ws.keys0,
+                                                            ws.vals0);
+    DTC_LAUNCH_CHECK("big_keys");
+    bool in_1 = false;
+    const int key_bits = colbits + bits_for(host.n_big - 1);
+    st = radix_sort_pairs(ws.keys0, ws.vals0, ws.keys1, ws.vals1, nb, key_bits, &in_1, ws.tmp,
                           stream);
     if (st != DTC_SUCCESS) return st;
-    // 3. Unique-column numbering of the sorted keys.
-    st = scan_key_changes(in_1 ? ws.keys1 : ws.keys0, ws.gscan, nnz, ws.tmp, stream);
+    st = scan_key_changes(in_1 ? ws.keys1 : ws.keys0, ws.gscan, nb, ws.tmp, stream);
     if (st != DTC_SUCCESS) return st;
+    k_header_set<<<1, 64, 0, stream>>>(ws.hdr, colbits, in_1 ? 1 : 0);
+    DTC_LAUNCH_CHECK("header_set");
+    k_big_counts<<<grid_for(host.n_big, 256), 256, 0, stream>>>(
+        ws.hdr, ws.gscan, ws.csb, ws.wmap, blk_w, block_partition, ws.wU, ws.cbb);
+    DTC_LAUNCH_CHECK("big_counts");
+    st = scan_i32(ws.cbb, ws.cbb, host.n_big, false, ws.tmp, stream);
+    if (st != DTC_SUCCESS) return st;
+    const unsigned grid = grid_for(nb, 256);
+    k_big_stage<<<grid, 256, 0, stream>>>(node_pointer, num_nodes, blk_h, blk_w, ws.hdr, ws.keys0,
+                                          ws.keys1, ws.vals0, ws.vals1, ws.gscan, ws.csb,
+                                          ws.wmap, ws.cbb, edge_to_column, win_stage(ws),
+                                          ws.bstart);
+    DTC_LAUNCH_CHECK("big_stage");
+    k_big_tile_ids<<<grid, 256, 0, stream>>>(node_pointer, edge_to_row, num_nodes, blk_h, blk_w,
+                                             ws.hdr, ws.keys0, ws.keys1, ws.vals0, ws.vals1,
+                                             ws.gscan, ws.csb, ws.wmap, ws.cbb, ws.bstart,
+                                             values, tcblocktile_id, tcblock_values,
+                                             tcblock_edge);
+    DTC_LAUNCH_CHECK("big_tile_ids");
   }
-  k_header_set<<<1, 64, 0, stream>>>(ws.hdr, colbits, in_1 ? 1 : 0);
-  DTC_LAUNCH_CHECK("header_set");
-  // 4. Blocks per window, window offsets.
-  k_window_counts<<<grid_for(W, 256), 256, 0, stream>>>(node_pointer, num_nodes, (int32_t)W, blk_h,
-                                                         blk_w, ws.gscan, block_partition,
-                                                         row_window_offset, ws.hdr);
-  DTC_LAUNCH_CHECK("window_counts");
+
+  // 4. Block count of the non-empty windows, window offsets.
+  k_sum_blocks<<<grid_for(W, 256), 256, 0, stream>>>(node_pointer, num_nodes, (int32_t)W, blk_h,
+                                                      block_partition, row_window_offset, ws.hdr);
+  DTC_LAUNCH_CHECK("sum_blocks");
   st = scan_i32(block_partition, row_window_offset + 1, W, true, ws.tmp, stream);
   if (st != DTC_SUCCESS) return st;
   k_copy_last<<<1, 64, 0, stream>>>(row_window_offset, (int32_t)W, ws.hdr);
   DTC_LAUNCH_CHECK("copy_last");
-  // 5. edgeToColumn.
-  if (nnz > 0) {
-    k_edge_to_column<<<grid_for(nnz, 256), 256, 0, stream>>>(
-        node_pointer, num_nodes, nnz, blk_h, ws.hdr, ws.keys0, ws.keys1, ws.vals0, ws.vals1,
-        ws.gscan, edge_to_column);
-    DTC_LAUNCH_CHECK("edge_to_column");
-  }
-  // 6. Block count (second host synchronisation).
-  DTC_HIP_CHECK(hipMemcpyAsync(&host, ws.hdr, 16, hipMemcpyDeviceToHost, stream));
+
+  // 5. Block count (second host synchronisation).
+  DTC_HIP_CHECK(hipMemcpyAsync(&host, ws.hdr, 64, hipMemcpyDeviceToHost, stream));
   DTC_HIP_CHECK(hipStreamSynchronize(stream));
+  if (host.flags & kBadClass)
+    return fail(DTC_ERROR_INCONSISTENT, "internal error: a window exceeded its class capacity");
   if (host.block_count != host.rwo_last)
     return fail(DTC_ERROR_INCONSISTENT,
                 "blockPartition must be zero-filled: row-window offsets end at " +
@@ -412,13 +1099,11 @@ dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *no
   return DTC_SUCCESS;
 }
 
-dtc_status_t dtc_metcf_build_blocks(const int32_t *node_pointer, const int32_t *edge_to_row,
+dtc_status_t dtc_metcf_build_blocks(const int32_t *node_pointer,
                                     const int32_t *row_window_offset, int32_t num_nodes,
                                     int32_t nnz, int32_t blk_h, int32_t blk_w,
-                                    int32_t block_count, const float *values,
-                                    int32_t *tcblock_rowid, uint8_t *tcblocktile_id,
+                                    int32_t block_count, int32_t *tcblock_rowid,
                                     int32_t *tcblock_offset, int32_t *sparse_atox,
-                                    float *tcblock_values, int32_t *tcblock_edge,
                                     void *workspace, size_t workspace_bytes,
                                     dtc_stream_t stream_) {
   clear_error();
@@ -428,8 +1113,6 @@ dtc_status_t dtc_metcf_build_blocks(const int32_t *node_pointer, const int32_t *
   if (block_count < 0) return fail(DTC_ERROR_INVALID_VALUE, "block_count must be >= 0");
   if (!node_pointer || !row_window_offset || !tcblock_offset)
     return fail(DTC_ERROR_INVALID_VALUE, "node_pointer, row_window_offset and tcblock_offset are required");
-  if (nnz > 0 && (!edge_to_row || !tcblocktile_id))
-    return fail(DTC_ERROR_INVALID_VALUE, "edge_to_row and tcblocktile_id are required when nnz > 0");
   if (block_count > 0 && (!tcblock_rowid || !sparse_atox))
     return fail(DTC_ERROR_INVALID_VALUE, "tcblock_rowid and sparse_atox are required");
   if ((nnz == 0) != (block_count == 0))
@@ -438,21 +1121,13 @@ dtc_status_t dtc_metcf_build_blocks(const int32_t *node_pointer, const int32_t *
   MetcfWs ws = carve(workspace, nnz, W);
   if (!workspace || workspace_bytes < ws.bytes)
     return fail(DTC_ERROR_WORKSPACE, "ME-TCF workspace needs " + std::to_string(ws.bytes) + " bytes");
-  if (nnz == 0) {
-    k_set_i32<<<1, 64, 0, stream>>>(tcblock_offset, 0);
-    DTC_LAUNCH_CHECK("set");
-    return DTC_SUCCESS;
-  }
-  const unsigned grid = grid_for(nnz, 256);
-  k_fill_blocks<<<grid, 256, 0, stream>>>(node_pointer, row_window_offset, num_nodes, nnz, blk_h,
-                                          blk_w, block_count, ws.hdr, ws.keys0, ws.keys1,
-                                          ws.gscan, tcblock_offset, tcblock_rowid, sparse_atox);
-  DTC_LAUNCH_CHECK("fill_blocks");
-  k_tile_ids<<<grid, 256, 0, stream>>>(node_pointer, row_window_offset, edge_to_row, num_nodes,
-                                       nnz, blk_h, blk_w, ws.hdr, ws.keys0, ws.keys1, ws.vals0,
-                                       ws.vals1, ws.gscan, tcblock_offset, values,
-                                       tcblocktile_id, tcblock_values, tcblock_edge);
-  DTC_LAUNCH_CHECK("tile_ids");
+  k_set_i32<<<1, 64, 0, stream>>>(tcblock_offset + block_count, nnz);
+  DTC_LAUNCH_CHECK("set");
+  if (block_count == 0) return DTC_SUCCESS;
+  k_blocks_out<<<grid_for(block_count, kOutBlocks), 256, 0, stream>>>(
+      node_pointer, num_nodes, (int32_t)W, blk_h, blk_w, block_count, win_stage(ws),
+      row_window_offset, tcblock_offset, tcblock_rowid, sparse_atox);
+  DTC_LAUNCH_CHECK("blocks_out");
   return DTC_SUCCESS;
 }
 

@@ -45,10 +45,9 @@ _SIGS = {
     "dtc_last_error": ([], ctypes.c_char_p),
     "dtc_version": ([], i32),
     "dtc_metcf_workspace_size": ([i32, i32, i32, i32, ctypes.POINTER(sz)], i32),
-    "dtc_metcf_build_windows": ([vp, vp, i32, i32, i32, i32, vp, vp, vp, vp, ctypes.POINTER(i32), vp, sz, vp],
-                                i32),
-    "dtc_metcf_build_blocks": ([vp, vp, vp, i32, i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp, sz, vp],
-                               i32),
+    "dtc_metcf_build_windows": ([vp, vp, i32, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp,
+                                 ctypes.POINTER(i32), vp, sz, vp], i32),
+    "dtc_metcf_build_blocks": ([vp, vp, i32, i32, i32, i32, i32, vp, vp, vp, vp, sz, vp], i32),
     "dtc_spmm_windows": ([vp, vp, vp, vp, vp, i32, i32, i32, i32, vp, i32, i32, vp,
                           ctypes.POINTER(SpmmOptions), vp], i32),
     "dtc_spmm_balanced_workspace_size": ([i32, ctypes.POINTER(sz)], i32),
@@ -85,22 +84,22 @@ class Library:
         return out.value
 
     def metcf_build_windows(self, edge_list, node_pointer, num_nodes, nnz, block_partition, edge_to_column,
-                            edge_to_row, row_window_offset, workspace, workspace_bytes, stream=0,
+                            edge_to_row, row_window_offset, tcblocktile_id, workspace, workspace_bytes,
+                            values=None, tcblock_values=None, tcblock_edge=None, stream=0,
                             blk_h=16, blk_w=8) -> int:
-        """Phase 1; returns the TC-block count."""
+        """Phase 1 (every output sized by the CSR); returns the TC-block count."""
         t = i32(0)
-        self.call("dtc_metcf_build_windows", edge_list, node_pointer, num_nodes, nnz, blk_h, blk_w,
-                  block_partition, edge_to_column, edge_to_row, row_window_offset, ctypes.byref(t),
-                  workspace, workspace_bytes, stream)
+        self.call("dtc_metcf_build_windows", edge_list, node_pointer, num_nodes, nnz, blk_h, blk_w, values,
+                  block_partition, edge_to_column, edge_to_row, row_window_offset, tcblocktile_id,
+                  tcblock_values, tcblock_edge, ctypes.byref(t), workspace, workspace_bytes, stream)
         return t.value
 
-    def metcf_build_blocks(self, node_pointer, edge_to_row, row_window_offset, num_nodes, nnz, block_count,
-                           tcblock_rowid, tcblocktile_id, tcblock_offset, sparse_atox, workspace,
-                           workspace_bytes, values=None, tcblock_values=None, tcblock_edge=None, stream=0,
+    def metcf_build_blocks(self, node_pointer, row_window_offset, num_nodes, nnz, block_count,
+                           tcblock_rowid, tcblock_offset, sparse_atox, workspace, workspace_bytes, stream=0,
                            blk_h=16, blk_w=8) -> None:
-        self.call("dtc_metcf_build_blocks", node_pointer, edge_to_row, row_window_offset, num_nodes, nnz,
-                  blk_h, blk_w, block_count, values, tcblock_rowid, tcblocktile_id, tcblock_offset,
-                  sparse_atox, tcblock_values, tcblock_edge, workspace, workspace_bytes, stream)
+        """Phase 2 (the outputs sized by the block count)."""
+        self.call("dtc_metcf_build_blocks", node_pointer, row_window_offset, num_nodes, nnz, blk_h, blk_w,
+                  block_count, tcblock_rowid, tcblock_offset, sparse_atox, workspace, workspace_bytes, stream)
 
     # ----------------------------------------------------------------- SpMM
     @staticmethod

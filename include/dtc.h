@@ -87,10 +87,11 @@ int32_t dtc_version(void);
  * Replaces DTCSpMM.preprocess_gpu (DTC-SpMM/DTCSpMM.cpp:313-359) and
  * seg_sort_dequ (DTC-SpMM/DTCSpMM_kernel.cu:349-407).  The reference sizes its
  * block-dependent outputs from a device-to-host read of the block count, so
- * the build is split in the same place: phase 1 computes the window arrays
- * and the block count, the caller allocates the block-sized outputs, phase 2
- * fills them.  The workspace carries the sorted edge keys from phase 1 to
- * phase 2 and must not be touched in between.
+ * the build is split in the same place: phase 1 fills every output sized by
+ * the CSR (windows or edges) and returns the block count, the caller
+ * allocates the block-sized outputs, phase 2 fills them.  The workspace
+ * carries per-window block data from phase 1 to phase 2 and must not be
+ * touched in between.
  *
  * Outputs are bit-identical to the reference's for the same inputs,
  * including empty windows (blockPartition left as passed), a partial last
@@ -108,32 +109,32 @@ dtc_status_t dtc_metcf_workspace_size(int32_t num_nodes, int32_t nnz, int32_t bl
  * callers do, scripts/DTCSpMM/run_DTC_SpMM.py:49-58): block_partition
  * [ceil(num_nodes / blk_h)], zero-filled by the caller; edge_to_column [nnz];
  * edge_to_row [nnz].  Out: row_window_offset [ceil(num_nodes / blk_h) + 1];
- * *block_count (host) = number of TC blocks T.  Synchronises the stream
- * twice (CSR check, then the block count), as the reference does
- * (DTCSpMM_kernel.cu:378-380, :384-387).
+ * tcblocktile_id [nnz] (uint8, TC-block order); *block_count (host) = number
+ * of TC blocks T.  Optional (NULL to skip, build extensions): values [nnz]
+ * (CSR order) -> tcblock_values [nnz] (TC-block order; 1.0 when values is
+ * NULL); tcblock_edge [nnz] receives the CSR edge index stored at each
+ * TC-block position.  Synchronises the stream twice (CSR check, then the
+ * block count), as the reference does (DTCSpMM_kernel.cu:378-380, :384-387).
  */
 dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *node_pointer,
                                      int32_t num_nodes, int32_t nnz, int32_t blk_h,
-                                     int32_t blk_w, int32_t *block_partition,
-                                     int32_t *edge_to_column, int32_t *edge_to_row,
-                                     int32_t *row_window_offset, int32_t *block_count,
+                                     int32_t blk_w, const float *values,
+                                     int32_t *block_partition, int32_t *edge_to_column,
+                                     int32_t *edge_to_row, int32_t *row_window_offset,
+                                     uint8_t *tcblocktile_id, float *tcblock_values,
+                                     int32_t *tcblock_edge, int32_t *block_count,
                                      void *workspace, size_t workspace_bytes,
                                      dtc_stream_t stream);
 
 /*
- * Phase 2 (same workspace, after phase 1).  Out: tcblock_rowid [T],
- * tcblocktile_id [nnz] (uint8), tcblock_offset [T + 1], sparse_atox
- * [T * blk_w].  Optional (NULL to skip, build extensions): values [nnz]
- * (CSR order) -> tcblock_values [nnz] (TC-block order); tcblock_edge [nnz]
- * receives the CSR edge index stored at each TC-block position.
+ * Phase 2 (same workspace, after phase 1; no host synchronisation).  Out:
+ * tcblock_rowid [T], tcblock_offset [T + 1], sparse_atox [T * blk_w].
  */
-dtc_status_t dtc_metcf_build_blocks(const int32_t *node_pointer, const int32_t *edge_to_row,
+dtc_status_t dtc_metcf_build_blocks(const int32_t *node_pointer,
                                     const int32_t *row_window_offset, int32_t num_nodes,
                                     int32_t nnz, int32_t blk_h, int32_t blk_w,
-                                    int32_t block_count, const float *values,
-                                    int32_t *tcblock_rowid, uint8_t *tcblocktile_id,
+                                    int32_t block_count, int32_t *tcblock_rowid,
                                     int32_t *tcblock_offset, int32_t *sparse_atox,
-                                    float *tcblock_values, int32_t *tcblock_edge,
                                     void *workspace, size_t workspace_bytes,
                                     dtc_stream_t stream);
 

@@ -122,8 +122,9 @@ def test_host_side_argument_errors(lib):
     st = lib.dtc_spmm_windows(dummy, dummy, dummy, dummy, None, 1, 1, 4, 100, dummy, 16, 8, dummy, None, None)
     assert st == 1
     # missing workspace for the ME-TCF build
-    st = lib.dtc_metcf_build_windows(dummy, dummy, 16, 4, 16, 8, dummy, dummy, dummy, dummy,
-                                     ctypes.byref(ctypes.c_int32()), None, 0, None)
+    st = lib.dtc_metcf_build_windows(dummy, dummy, 16, 4, 16, 8, None, dummy, dummy, dummy, dummy,
+                                     dummy, None, None, ctypes.byref(ctypes.c_int32()), None,
+                                     ctypes.c_size_t(0), None)
     assert st == 3
     # balanced launches without their workspace (persistent or not), or with
     # one smaller than the size query reports
@@ -148,21 +149,23 @@ def test_host_side_argument_errors(lib):
             o = Opts(0, 0, 0, 2, pers, 0, det)
             need = ctypes.c_size_t()
             assert lib.dtc_spmm_balanced_workspace_size_ex(4, 8, ctypes.byref(o), ctypes.byref(need)) == 0
+            # sizes as size_t: a plain int would leave the upper half of the
+            # 64-bit argument slot undefined (no argtypes on this handle)
             st = lib.dtc_spmm_balanced(dummy, dummy, dummy, dummy, None, 4, 8, 16, dummy, 16, 8, dummy,
-                                       ctypes.byref(o), None, 0, None)
+                                       ctypes.byref(o), None, ctypes.c_size_t(0), None)
             assert st == 3
             st = lib.dtc_spmm_balanced(dummy, dummy, dummy, dummy, None, 4, 8, 16, dummy, 16, 8, dummy,
-                                       ctypes.byref(o), dummy, need.value - 1, None)
+                                       ctypes.byref(o), dummy, ctypes.c_size_t(need.value - 1), None)
             assert st == 3 and b"workspace" in lib.dtc_last_error()
     o = Opts(0, 0, 0, 2, 0, 0, 2)  # deterministic must be 0 or 1
     st = lib.dtc_spmm_balanced(dummy, dummy, dummy, dummy, None, 4, 8, 16, dummy, 16, 8, dummy,
-                               ctypes.byref(o), dummy, 1 << 20, None)
+                               ctypes.byref(o), dummy, ctypes.c_size_t(1 << 20), None)
     assert st == 1
     # the cut span is a non-negative multiple of the chunk
     for span in (-4, 3):
         o = Opts(0, 0, 0, 2, 0, 0, 0, span)
         st = lib.dtc_spmm_balanced(dummy, dummy, dummy, dummy, None, 4, 8, 16, dummy, 16, 8, dummy,
-                                   ctypes.byref(o), dummy, 1 << 20, None)
+                                   ctypes.byref(o), dummy, ctypes.c_size_t(1 << 20), None)
         assert st == 1 and b"cut_span_blocks" in lib.dtc_last_error()
 
 

@@ -52,16 +52,16 @@ def test_capi_metcf_and_spmm():
         e2c = torch.empty(nnz, dtype=i32, device="cuda")
         e2r = torch.empty(nnz, dtype=i32, device="cuda")
         rwo = torch.empty(W + 1, dtype=i32, device="cuda")
-        T = lib.metcf_build_windows(col_d.data_ptr(), rp_d.data_ptr(), M, nnz, bp.data_ptr(), e2c.data_ptr(),
-                                    e2r.data_ptr(), rwo.data_ptr(), ws.data_ptr(), ws_bytes, s)
-        rowid = torch.empty(T, dtype=i32, device="cuda")
         tile = torch.empty(nnz, dtype=u8, device="cuda")
+        tcv = torch.empty(nnz, dtype=f32, device="cuda")
+        T = lib.metcf_build_windows(col_d.data_ptr(), rp_d.data_ptr(), M, nnz, bp.data_ptr(), e2c.data_ptr(),
+                                    e2r.data_ptr(), rwo.data_ptr(), tile.data_ptr(), ws.data_ptr(), ws_bytes,
+                                    values=vals_d.data_ptr(), tcblock_values=tcv.data_ptr(), stream=s)
+        rowid = torch.empty(T, dtype=i32, device="cuda")
         off = torch.empty(T + 1, dtype=i32, device="cuda")
         atox = torch.empty(8 * T, dtype=i32, device="cuda")
-        tcv = torch.empty(nnz, dtype=f32, device="cuda")
-        lib.metcf_build_blocks(rp_d.data_ptr(), e2r.data_ptr(), rwo.data_ptr(), M, nnz, T, rowid.data_ptr(),
-                               tile.data_ptr(), off.data_ptr(), atox.data_ptr(), ws.data_ptr(), ws_bytes,
-                               values=vals_d.data_ptr(), tcblock_values=tcv.data_ptr(), stream=s)
+        lib.metcf_build_blocks(rp_d.data_ptr(), rwo.data_ptr(), M, nnz, T, rowid.data_ptr(), off.data_ptr(),
+                               atox.data_ptr(), ws.data_ptr(), ws_bytes, stream=s)
         torch.cuda.synchronize()
         ref = oracle.metcf(rowptr, col, M, values=vals)
         assert T == ref["block_count"], name
@@ -112,12 +112,13 @@ def test_capi_errors_are_status_codes():
     ws_bytes = lib.metcf_workspace_size(3, 3)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
     bufs = [torch.zeros(n, dtype=torch.int32, device="cuda") for n in (1, 3, 3, 2)]
+    tile = torch.zeros(3, dtype=torch.uint8, device="cuda")
     say("capi invalid CSR")
     with pytest.raises(DTCError) as e:
         lib.metcf_build_windows(col_d.data_ptr(), rp_d.data_ptr(), 3, 3, *[b.data_ptr() for b in bufs],
-                                ws.data_ptr(), ws_bytes, s)
+                                tile.data_ptr(), ws.data_ptr(), ws_bytes, stream=s)
     assert e.value.status == 4 and "nodePointer" in e.value.detail
     with pytest.raises(DTCError) as e:
         lib.metcf_build_windows(col_d.data_ptr(), rp_d.data_ptr(), 3, 3, *[b.data_ptr() for b in bufs],
-                                ws.data_ptr(), ws_bytes - 1, s)
+                                tile.data_ptr(), ws.data_ptr(), ws_bytes - 1, stream=s)
     assert e.value.status == 3

@@ -1,4 +1,13 @@
-"""GPU ME-TCF build vs the CPU oracle: bit-exact on every output array."""
+"""GPU ME-TCF build vs the CPU oracle: bit-exact on every output array.
+
+Most tests run under every window routing of the build (DTC_METCF_PATH):
+"auto" (by entry count: one LDS workgroup per window up to 16,384 entries,
+the device-wide radix sort beyond), "global" (every window through the
+radix sort) and "lds0" ... "lds4" (every window that fits the 1-, 2-, 4-,
+8- or 16-wave workgroup shape through it, the rest through the radix
+sort)."""
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -15,8 +24,18 @@ def ext():
     return load_extension()
 
 
+PATHS = ["auto", "global", "lds0", "lds1", "lds2", "lds3", "lds4"]
+
+
+@pytest.fixture(params=PATHS)
+def path(request, monkeypatch):
+    monkeypatch.setenv("DTC_METCF_PATH", request.param)
+    return request.param
+
+
 def check(ext, rowptr, col, M, bh=16, bw=8, values=None, ctx=""):
-    say("metcf", ctx, f"M={M} nnz={col.shape[0]} bh={bh} bw={bw}")
+    say("metcf", ctx, f"M={M} nnz={col.shape[0]} bh={bh} bw={bw}",
+        "path=" + os.environ.get("DTC_METCF_PATH", "auto"))
     g = gpu_metcf(ext, rowptr, col, M, bh, bw, values=values)
     o = oracle.metcf(rowptr, col, M, bh, bw, values=values)
     assert_metcf_equal(g, o, ctx=ctx)
@@ -27,7 +46,7 @@ def check(ext, rowptr, col, M, bh=16, bw=8, values=None, ctx=""):
     return g
 
 
-def test_fixture_18(ext):
+def test_fixture_18(ext, path):
     rowptr, col, M, fx = load_fixture_18()
     g = check(ext, rowptr, col, M, ctx="fixture18")
     for k, v in fx["expected"].items():
@@ -38,7 +57,7 @@ def test_fixture_18(ext):
 
 
 @pytest.mark.parametrize("bh,bw", [(16, 8), (16, 16), (8, 8), (16, 4), (16, 32), (1, 8), (32, 8)])
-def test_random_small(ext, bh, bw):
+def test_random_small(ext, path, bh, bw):
     rng = np.random.default_rng(50 + bh * 7 + bw)
     for case in range(12):
         M = int(rng.integers(1, 90))
@@ -57,19 +76,19 @@ def test_c1_uniform(ext):
 
 
 @pytest.mark.parametrize("scale", [10, 14])
-def test_rmat(ext, scale):
+def test_rmat(ext, path, scale):
     from dtc.graphs import rmat
     a = rmat(scale, 16, seed=1)
     check(ext, a.rowptr, a.col, a.num_rows, values=a.values, ctx=f"rmat{scale}")
 
 
-def test_sbm_hubs_small(ext):
+def test_sbm_hubs_small(ext, path):
     from dtc.graphs import sbm_hubs
     a = sbm_hubs(20000, communities=20, avg_per_row=40, hubs=6, hub_min=3000, hub_max=8000, seed=2)
     check(ext, a.rowptr, a.col, a.num_rows, values=a.values, ctx="sbm")
 
 
-def test_banded_and_block_diagonal(ext):
+def test_banded_and_block_diagonal(ext, path):
     from dtc.graphs import banded, block_diagonal
     a = banded(3000, 40, seed=1)
     check(ext, a.rowptr, a.col, a.num_rows, ctx="banded")
@@ -77,7 +96,7 @@ def test_banded_and_block_diagonal(ext):
     check(ext, b.rowptr, b.col, b.num_rows, ctx="blockdiag")
 
 
-def test_window_beyond_reference_smem_limit(ext):
+def test_window_beyond_reference_smem_limit(ext, path):
     # one window with more than 16,383 TC blocks overflows the reference's
     # 128 KB shared-memory opt-in (DTCSpMM_kernel.cu:311-325); no limit here.
     rng = np.random.default_rng(9)
@@ -89,13 +108,13 @@ def test_window_beyond_reference_smem_limit(ext):
     assert g["block_count"] > 16383
 
 
-def test_unsorted_rows_and_duplicates(ext):
+def test_unsorted_rows_and_duplicates(ext, path):
     rng = np.random.default_rng(11)
     rowptr, col = random_csr(rng, 1000, 500, 20.0, sorted_rows=False, dup_frac=0.5)
     check(ext, rowptr, col, 1000, ctx="unsorted+dups")
 
 
-def test_empty_and_degenerate(ext):
+def test_empty_and_degenerate(ext, path):
     for M in [0, 1, 15, 16, 17, 100]:
         rowptr = np.zeros(M + 1, np.int32)
         col = np.zeros(0, np.int32)
@@ -167,7 +186,7 @@ def test_metcf_cache_roundtrip(ext, tmp_path):
 
 
 @pytest.mark.parametrize("bw", [1, 8])
-def test_mixed_window_sizes(ext, bw):
+def test_mixed_window_sizes(ext, path, bw):
     """Windows of very different sizes side by side: one of 65,000 edges over
     columns up to 3,000,000 (with 1-column blocks, about 60,000 blocks in one
     window), one of 6,000 edges over 50 columns, one of 3,000 edges, and many
@@ -187,7 +206,7 @@ def test_mixed_window_sizes(ext, bw):
 
 
 @pytest.mark.parametrize("per_window", [4, 8])
-def test_many_short_windows(ext, per_window):
+def test_many_short_windows(ext, path, per_window):
     """Tens of thousands of windows of a few edges each (random rows of the
     window, duplicates dropped), so that thousands of consecutive windows
     share every stretch of 2,048 edges."""
