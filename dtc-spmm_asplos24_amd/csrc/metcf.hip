@@ -46,10 +46,12 @@ namespace {
 // 64 * waves * entries.
 constexpr int kWinH = 16;  // rows per window on the LDS path (the reference's BLK_H)
 constexpr int kWinW = 8;   // columns per TC block on the LDS path (BLK_W)
-constexpr int kClasses = 5;
-constexpr int kClassWaves[kClasses] = {1, 2, 4, 8, 16};
-constexpr int kClassItems[kClasses] = {8, 16, 16, 16, 16};
-constexpr int kClassCap[kClasses] = {64 * 1 * 8, 64 * 2 * 16, 64 * 4 * 16, 64 * 8 * 16,
+// Class 0 (at most 64 entries) is one wave per window with bitonic sorts
+// (k_window_tiny); classes 1-5 are k_window_sort.
+constexpr int kClasses = 6;
+constexpr int kClassWaves[kClasses] = {1, 1, 2, 4, 8, 16};
+constexpr int kClassItems[kClasses] = {1, 8, 16, 16, 16, 16};
+constexpr int kClassCap[kClasses] = {64, 64 * 1 * 8, 64 * 2 * 16, 64 * 4 * 16, 64 * 8 * 16,
                                      64 * 16 * 16};
 
 // Routing of windows (environment DTC_METCF_PATH): by size, every window
@@ -59,16 +61,12 @@ constexpr int kPathGlobal = 1;
 constexpr int kPathClass0 = 2;
 
 struct MetcfHeader {
-  int32_t flags;        // CSR check bits, see kBad*
-  int32_t max_col;      // largest column index
-  int32_t block_count;  // sum of blocks over non-empty windows
-  int32_t rwo_last;     // row_window_offset[W]
-  int32_t colbits;      // bits used for the column in the global-path key
-  int32_t sorted_in_1;  // global-path sorted pairs live in buffer 1
-  int32_t n_big;        // windows on the global path
-  int32_t nnz_big;      // their entries
-  int32_t cls_count[kClasses];  // windows per LDS class
-  int32_t pad[51];
+  int32_t flags;    // CSR check bits, see kBad*
+  int32_t max_col;  // largest column index
+  int32_t nnz_big;  // entries of the global-path windows
+  int32_t pad0;
+  int32_t cls_count[kClasses + 1];  // windows per LDS class; [kClasses]: global-path windows
+  int32_t pad[53];
 };
 static_assert(sizeof(MetcfHeader) == 256, "header size");
 
@@ -77,6 +75,7 @@ constexpr int kBadLast = 2;      // rowptr[M] != nnz
 constexpr int kBadOrder = 4;     // rowptr decreases
 constexpr int kBadColumn = 8;    // negative column index
 constexpr int kBadClass = 16;    // a window larger than its class (internal)
+constexpr int kBadPartition = 32;  // blockPartition of an empty window is not zero
 
 struct MetcfWs {
   MetcfHeader *hdr;
@@ -86,11 +85,8 @@ struct MetcfWs {
   int32_t *boff;            // staged block data for phase 2 (see WinStage)
   int32_t *dcol;
   int32_t *wU;
-  int32_t *lists;           // [kClasses][W] windows of each LDS class
-  int32_t *widx;            // inclusive count of global-path windows up to w
-  int32_t *wcs;             // inclusive count of their entries up to w
-  int32_t *wmap;            // compacted window -> window
-  int32_t *wcls;            // class of each window: -1 empty, LDS class, kClasses global
+  int32_t *lists;           // [kClasses + 1][W] windows of each class; the last row
+                            // (global path) maps compacted window -> window
   int32_t *bstart;          // global path: first compacted entry of each of its blocks
   int32_t *cbb;             // global path: first block of each compacted window [n_big + 1]
   int32_t *csb;             // compacted window -> first compacted entry [n_big + 1]
@@ -112,11 +108,7 @@ MetcfWs carve(void *ws, int64_t nnz, int64_t W) {
   w.boff = c.take<int32_t>(n);
   w.dcol = c.take<int32_t>(n);
   w.wU = c.take<int32_t>(nw);
-  w.lists = c.take<int32_t>(kClasses * nw);
-  w.widx = c.take<int32_t>(nw);
-  w.wcs = c.take<int32_t>(nw);
-  w.wmap = c.take<int32_t>(nw);
-  w.wcls = c.take<int32_t>(nw);
+  w.lists = c.take<int32_t>((kClasses + 1) * nw);
   w.bstart = c.take<int32_t>(n);
   w.cbb = c.take<int32_t>(nw + 1);
   w.csb = c.take<int32_t>(nw + 1);
@@ -149,22 +141,11 @@ __global__ void k_header_init(MetcfHeader *h) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     h->flags = 0;
     h->max_col = 0;
-    h->block_count = 0;
-    h->rwo_last = 0;
-    h->colbits = 0;
-    h->sorted_in_1 = 0;
-    h->n_big = 0;
     h->nnz_big = 0;
-    for (int c = 0; c < kClasses; ++c) h->cls_count[c] = 0;
+    for (int c = 0; c <= kClasses; ++c) h->cls_count[c] = 0;
   }
 }
 
-__global__ void k_header_set(MetcfHeader *h, int colbits, int sorted_in_1) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    h->colbits = colbits;
-    h->sorted_in_1 = sorted_in_1;
-  }
-}
 
 // Checks rowptr[0] == 0, rowptr[M] == nnz, rowptr non-decreasing, columns
 // >= 0, and finds the largest column.  Columns are read 16 bytes per lane
@@ -229,22 +210,26 @@ __device__ __forceinline__ int window_end(const int32_t *rowptr, int w, int blk_
   return rowptr[min((int64_t)w * blk_h + blk_h, (int64_t)M)];
 }
 
-// Window classification.  Non-empty windows that fit an LDS class are
-// appended to that class's list (one global atomic per workgroup and class;
-// the order of a list does not affect any output); the others are flagged for
-// the global path (widx = 1, wcs = E), both prefix-summed afterwards.
+// Window classification.  Each non-empty window is appended to the list of
+// its class (one global atomic per workgroup and class; the order of a list
+// does not affect any output): an LDS class, or the global path (list
+// kClasses, whose entries are summed into nnz_big).  Empty windows must have
+// blockPartition 0 (the caller zero-fills it; empty windows are not written).
+// Also writes row_window_offset[0] = 0.
 __global__ __launch_bounds__(256) void k_classify(const int32_t *rowptr, int32_t M, int32_t W,
-                                                  int32_t blk_h, int mode, int32_t *lists,
-                                                  int32_t *widx, int32_t *wcs, int32_t *wcls,
-                                                  MetcfHeader *h) {
-  __shared__ int s_cnt[kClasses], s_base[kClasses];
+                                                  int32_t blk_h, int mode, const int32_t *bp,
+                                                  int32_t *lists, int32_t *rwo, MetcfHeader *h) {
+  constexpr int NC = kClasses + 1;
+  __shared__ int s_cnt[NC], s_base[NC], s_big;
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
-  if (threadIdx.x < kClasses) s_cnt[threadIdx.x] = 0;
+  if (threadIdx.x < NC) s_cnt[threadIdx.x] = 0;
+  if (threadIdx.x == 0) s_big = 0;
   __syncthreads();
-  int cls = -1;
+  int cls = -1, E = 0;
+  bool bad = false;
   if (w < W) {
-    const int E = window_end(rowptr, w, blk_h, M) - window_begin(rowptr, w, blk_h, M);
+    E = window_end(rowptr, w, blk_h, M) - window_begin(rowptr, w, blk_h, M);
     if (E > 0) {
       cls = kClasses;
       if (mode == kPathAuto) {
@@ -254,15 +239,18 @@ __global__ __launch_bounds__(256) void k_classify(const int32_t *rowptr, int32_t
         const int c = mode - kPathClass0;
         if (E <= kClassCap[c]) cls = c;
       }
+    } else {
+      bad = bp[w] != 0;
     }
-    widx[w] = cls == kClasses ? 1 : 0;
-    wcs[w] = cls == kClasses ? E : 0;
-    wcls[w] = cls;
   }
-  uint64_t mask[kClasses];
-  int woff[kClasses];
+  if (w == 0) rwo[0] = 0;
+  if (__ballot(bad) && lane == 0) atomicOr(&h->flags, kBadPartition);
+  const int big_e = wave_sum(cls == kClasses ? E : 0);
+  if (lane == 0 && big_e) atomicAdd(&s_big, big_e);
+  uint64_t mask[NC];
+  int woff[NC];
 #pragma unroll
-  for (int c = 0; c < kClasses; ++c) {
+  for (int c = 0; c < NC; ++c) {
     mask[c] = __ballot(cls == c);
     woff[c] = 0;
     if (mask[c]) {
@@ -272,11 +260,12 @@ __global__ __launch_bounds__(256) void k_classify(const int32_t *rowptr, int32_t
     }
   }
   __syncthreads();
-  if (threadIdx.x < kClasses)
+  if (threadIdx.x < NC)
     s_base[threadIdx.x] = s_cnt[threadIdx.x] ? atomicAdd(&h->cls_count[threadIdx.x], s_cnt[threadIdx.x]) : 0;
+  if (threadIdx.x == 0 && s_big) atomicAdd(&h->nnz_big, s_big);
   __syncthreads();
 #pragma unroll
-  for (int c = 0; c < kClasses; ++c) {
+  for (int c = 0; c < NC; ++c) {
     if (cls == c) {
       const int below = (int)__builtin_amdgcn_mbcnt_hi(
           (uint32_t)(mask[c] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask[c], 0u));
@@ -285,23 +274,59 @@ __global__ __launch_bounds__(256) void k_classify(const int32_t *rowptr, int32_t
   }
 }
 
-// From the inclusive counts: wmap[w'] = w and csb[w'] = first compacted entry
-// of the w'-th global-path window; the totals go to the header.
-__global__ __launch_bounds__(256) void k_big_map(int32_t W, const int32_t *widx,
-                                                 const int32_t *wcs, int32_t *wmap,
-                                                 int32_t *csb, MetcfHeader *h) {
-  const int w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= W) return;
-  const int prev = w ? widx[w - 1] : 0;
-  if (widx[w] != prev) {
-    wmap[prev] = w;
-    csb[prev] = w ? wcs[w - 1] : 0;
+// Exclusive prefix sum by one 1,024-thread workgroup, 16 items per thread
+// per step: store(i, sum of load(k) for k < i) for i in [0, n), in order of i
+// for each thread's items; returns the total.  Every thread must call it.
+template <class Load, class Store>
+__device__ __forceinline__ int wg1024_exclusive_scan(Load load, Store store, int n, int *s_tmp) {
+  constexpr int NT = 1024, IT = 16;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  int carry = 0;
+  for (int base = 0; base < n; base += NT * IT) {
+    int v[IT];
+    int sum = 0;
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int i = base + tid * IT + it;
+      v[it] = i < n ? load(i) : 0;
+      sum += v[it];
+    }
+    const int inc = wave_inclusive_sum(sum);
+    if (lane == 63) s_tmp[wave] = inc;
+    __syncthreads();
+    int pre = 0, tot = 0;
+#pragma unroll
+    for (int j = 0; j < NT / 64; ++j) {
+      const int t = s_tmp[j];
+      pre += j < wave ? t : 0;
+      tot += t;
+    }
+    __syncthreads();
+    int run = carry + pre + inc - sum;
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int i = base + tid * IT + it;
+      if (i < n) store(i, run);
+      run += v[it];
+    }
+    carry += tot;
   }
-  if (w == W - 1) {
-    h->n_big = widx[W - 1];
-    h->nnz_big = wcs[W - 1];
-    csb[widx[W - 1]] = wcs[W - 1];
-  }
+  return carry;
+}
+
+// First compacted entry of each global-path window: csb[k] = entries of the
+// windows listed before k; csb[n_big] = all of them.
+__global__ __launch_bounds__(1024) void k_big_offsets(const int32_t *rowptr, int32_t M,
+                                                      int32_t blk_h, const int32_t *wmap,
+                                                      int32_t n_big, int32_t *csb) {
+  __shared__ int s_tmp[16];
+  const int total = wg1024_exclusive_scan(
+      [&](int i) {
+        const int w = wmap[i];
+        return window_end(rowptr, w, blk_h, M) - window_begin(rowptr, w, blk_h, M);
+      },
+      [&](int i, int x) { csb[i] = x; }, n_big, s_tmp);
+  if (threadIdx.x == 0) csb[n_big] = total;
 }
 
 // ------------------------------------------------------------- LDS path
@@ -332,62 +357,75 @@ __device__ __forceinline__ int lanes_below(uint64_t m) {
                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-// s_cnt[v][d] holds the number of entries of digit d among wave v's entries;
-// replaced by the position of wave v's first entry of digit d in the
-// digit-sorted order (digit-major, then wave).  Every thread must call it.
-template <int WAVES>
-__device__ __forceinline__ void digit_offsets(int (*s_cnt)[256], int *s_scan4) {
-  constexpr int NT = 64 * WAVES;
-  constexpr int DPT = NT >= 256 ? 1 : 256 / NT;  // digits per thread
-  constexpr int ACTIVE = 256 / DPT;              // threads holding digits: 64 or 256
+// cnt[row * NDIG + d] holds the number of entries of digit d in counter row
+// `row` (one row per wave, or a single workgroup-wide row); replaced by the
+// position of that row's first entry of digit d in the digit-sorted order
+// (digit-major, then row).  Every thread of the NT-thread workgroup must call
+// it.
+template <int NT, int NDIG, int NROWS>
+__device__ __forceinline__ void counts_to_offsets(int *cnt, int *s_scan) {
+  constexpr int DPT = NDIG >= NT ? NDIG / NT : 1;  // digits per thread
+  constexpr int ACTIVE = NDIG / DPT;               // threads holding digits
   const int tid = threadIdx.x;
-  int loc[DPT];
   int sum = 0;
   if (tid < ACTIVE) {
-#pragma unroll
-    for (int q = 0; q < DPT; ++q) {
-      const int d = tid * DPT + q;
-      int run = 0;
-#pragma unroll
-      for (int vv = 0; vv < WAVES; ++vv) {
-        const int c = s_cnt[vv][d];
-        s_cnt[vv][d] = run;
-        run += c;
-      }
-      loc[q] = sum;
-      sum += run;
-    }
+#pragma unroll 4
+    for (int q = 0; q < DPT; ++q)
+#pragma unroll 4
+      for (int vv = 0; vv < NROWS; ++vv) sum += cnt[vv * NDIG + tid * DPT + q];
   }
   const int inc = wave_inclusive_sum(sum);
-  int excl = inc - sum;
+  int base = inc - sum;
   if (ACTIVE > 64) {
-    if ((tid & 63) == 63 && tid < ACTIVE) s_scan4[tid >> 6] = inc;
+    if ((tid & 63) == 63 && tid < ACTIVE) s_scan[tid >> 6] = inc;
     __syncthreads();
     const int wv = tid >> 6;
 #pragma unroll
-    for (int j = 0; j < ACTIVE / 64; ++j) excl += j < wv ? s_scan4[j] : 0;
+    for (int j = 0; j < ACTIVE / 64; ++j) base += j < wv ? s_scan[j] : 0;
   }
   if (tid < ACTIVE) {
-#pragma unroll
+#pragma unroll 2
     for (int q = 0; q < DPT; ++q) {
-      const int d = tid * DPT + q;
-#pragma unroll
-      for (int vv = 0; vv < WAVES; ++vv) s_cnt[vv][d] += excl + loc[q];
+#pragma unroll 4
+      for (int vv = 0; vv < NROWS; ++vv) {
+        const int i = vv * NDIG + tid * DPT + q;
+        const int c = cnt[i];
+        cnt[i] = base;
+        base += c;
+      }
     }
   }
   __syncthreads();
 }
 
+// Digit widths of the window sort per workgroup shape: the first pass (not
+// stable, one workgroup-wide counter row) takes D1 bits, the later ones
+// (stable, one counter row per wave) D2 bits, so that a 20-bit column span
+// takes two passes in the 2- to 16-wave shapes while the counters stay small
+// enough for four waves per SIMD (the 1-wave shape, for windows of at most
+// 512 entries, keeps 8-bit digits: its fixed cost per digit dominates).  The
+// 1-, 2- and 4-wave shapes move key and index in one step through a 32-bit
+// and a 16-bit buffer; the larger ones through one 16-bit buffer.
+constexpr int win_d1(int waves) { return waves == 1 ? 8 : waves <= 4 ? 11 : 12; }
+constexpr int win_d2(int waves) { return waves == 1 ? 8 : waves <= 8 ? 9 : 8; }
+constexpr int win_cnt(int waves) {
+  return (1 << win_d1(waves)) > (waves << win_d2(waves)) ? (1 << win_d1(waves))
+                                                          : (waves << win_d2(waves));
+}
+constexpr bool win_wide(int waves, int items) { return 64 * waves * items <= 4096; }
+
 // Phase 1 of the LDS path, one workgroup per window of the class list.
 // Local CSR entry q is held by wave q / (64 ITEMS), slot (q / 64) % ITEMS,
 // lane q % 64; after the sort the same layout holds sorted position p.
-//   sort   stable LSD radix sort of (column - window minimum, q) over the
-//          window's column span, 8-bit digits: each wave ranks its entries
-//          slot by slot (one ballot per digit bit gives the lanes of equal
-//          digit, added to per-wave LDS counters), digit_offsets turns the
-//          counts into positions, and key and index move to them through
-//          LDS (one 32-bit + one 16-bit buffer, or for the 16-wave shape one
-//          16-bit buffer in two or three pieces);
+//   sort   LSD radix sort of (column - window minimum, q) over the window's
+//          column span: a first pass of D1 bits ranked by LDS atomics on
+//          one workgroup-wide counter row (it need not be stable: there is
+//          no earlier order to keep, and no output depends on the order of
+//          entries of equal column), then stable passes of D2 bits in which
+//          each wave ranks its entries slot by slot (one ballot per digit
+//          bit gives the lanes of equal digit, added to per-wave LDS
+//          counters); counts_to_offsets turns the counts into positions,
+//          and key and index move to them through LDS;
 //   ranks  a sorted entry is new when its column differs from its
 //          predecessor's; rank = (new entries up to it) - 1 is its
 //          edgeToColumn, U = number of distinct columns;
@@ -404,17 +442,19 @@ __global__ __launch_bounds__(64 * WAVES) void k_window_sort(
     const int32_t *__restrict__ list, const float *__restrict__ values, int32_t *__restrict__ e2r,
     int32_t *__restrict__ e2c, int32_t *__restrict__ bp, uint8_t *__restrict__ tile,
     float *__restrict__ tcvalues, int32_t *__restrict__ tcedge, WinStage st, MetcfHeader *h) {
-  constexpr int CAP = 64 * WAVES * ITEMS;
-  // key and index move in one step (32-bit + 16-bit buffers) when they fit
-  constexpr bool kWide = CAP * 6 + WAVES * 1024 + CAP / 4 <= 61440;
+  constexpr int NT = 64 * WAVES;
+  constexpr int CAP = NT * ITEMS;
+  constexpr bool kWide = win_wide(WAVES, ITEMS);
+  constexpr int D1 = win_d1(WAVES), D2 = win_d2(WAVES);
+  constexpr int N1 = 1 << D1, N2 = 1 << D2;
   constexpr int kRaw = kWide ? CAP + CAP / 2 : CAP / 2;
   __shared__ uint32_t s_raw[kRaw];
-  __shared__ int s_cnt[WAVES][256];
+  __shared__ int s_cnt[win_cnt(WAVES)];
   __shared__ uint16_t s_bs[CAP / kWinW + 1];
   __shared__ int s_rp[kWinH + 1];
   __shared__ int s_min[WAVES], s_max[WAVES], s_wtot[WAVES];
   __shared__ uint32_t s_last[WAVES];
-  __shared__ int s_scan4[4];
+  __shared__ int s_scan[16];
   uint32_t *s_k = s_raw;                                                 // kWide only
   uint16_t *s_x = reinterpret_cast<uint16_t *>(kWide ? s_raw + CAP : s_raw);
   const int tid = threadIdx.x, v = tid >> 6, lane = tid & 63;
@@ -472,50 +512,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_window_sort(
   for (int k = 0; k < ITEMS; ++k)
     if (vbase + k * 64 + lane < E) key[k] -= (uint32_t)cmin;
 
-  for (int shift = 0; shift < bits; shift += 8) {
-    const int nbits = min(8, bits - shift);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) s_cnt[v][j * 64 + lane] = 0;
-    wave_lds_sync();
-    if (shift == 0) {
-      // The first pass need not be stable (there is no earlier order to
-      // keep; entries of equal column may end in any order, which no output
-      // depends on): rank by LDS atomics on the wave's counters.
-#pragma unroll
-      for (int k = 0; k < ITEMS; ++k) {
-        if (vbase + k * 64 < E) {  // wave-uniform
-          const bool valid = vbase + k * 64 + lane < E;
-          const int d = (int)(key[k] & 255u);
-          const int r = valid ? atomicAdd(&s_cnt[v][d], 1) : 0;
-          ix[k] = (ix[k] & 0xffffu) | ((uint32_t)r << 16);
-        }
-      }
-    } else {
-#pragma unroll
-    for (int k = 0; k < ITEMS; ++k) {
-      if (vbase + k * 64 < E) {  // wave-uniform
-        const bool valid = vbase + k * 64 + lane < E;
-        const int d = (int)((key[k] >> shift) & 255u);
-        uint64_t peers = __ballot(valid);
-        for (int bit = 0; bit < nbits; ++bit) {
-          const bool b = (d >> bit) & 1;
-          const uint64_t m = __ballot(b);
-          peers &= b ? m : ~m;
-        }
-        const int r = lanes_below(peers);
-        const int c = valid ? s_cnt[v][d] : 0;
-        ix[k] = (ix[k] & 0xffffu) | ((uint32_t)(c + r) << 16);
-        wave_lds_sync();
-        if (valid && r == 0) s_cnt[v][d] = c + __popcll(peers);
-        wave_lds_sync();
-      }
-    }
-    }
-    __syncthreads();
-    digit_offsets<WAVES>(s_cnt, s_scan4);
-#pragma unroll
-    for (int k = 0; k < ITEMS; ++k)
-      if (vbase + k * 64 + lane < E) ix[k] += (uint32_t)s_cnt[v][(key[k] >> shift) & 255u] << 16;
+  // Moves key and index of every entry to the position in the high half of
+  // its ix word.
+  auto move = [&]() {
     if (kWide) {
 #pragma unroll
       for (int k = 0; k < ITEMS; ++k) {
@@ -573,6 +572,63 @@ __global__ __launch_bounds__(64 * WAVES) void k_window_sort(
       }
       __syncthreads();
     }
+  };
+
+  int shift = 0;
+  if (bits > 0) {
+    // first pass: low D1 bits, ranks from atomics on one counter row
+    for (int i = tid; i < N1; i += NT) s_cnt[i] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      if (vbase + k * 64 < E) {  // wave-uniform
+        const bool valid = vbase + k * 64 + lane < E;
+        const int r = valid ? atomicAdd(&s_cnt[key[k] & (N1 - 1)], 1) : 0;
+        ix[k] = (ix[k] & 0xffffu) | ((uint32_t)r << 16);
+      }
+    }
+    __syncthreads();
+    counts_to_offsets<NT, N1, 1>(s_cnt, s_scan);
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k)
+      if (vbase + k * 64 + lane < E) ix[k] += (uint32_t)s_cnt[key[k] & (N1 - 1)] << 16;
+    move();
+    shift = D1;
+  }
+  for (; shift < bits; shift += D2) {
+    // stable pass: D2 bits from `shift`, ranks from ballots, one counter
+    // row per wave
+    const int nbits = min(D2, bits - shift);
+    int *row = s_cnt + v * N2;
+#pragma unroll
+    for (int j = 0; j < N2 / 64; ++j) row[j * 64 + lane] = 0;
+    wave_lds_sync();
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      if (vbase + k * 64 < E) {  // wave-uniform
+        const bool valid = vbase + k * 64 + lane < E;
+        const int d = (int)((key[k] >> shift) & (uint32_t)(N2 - 1));
+        uint64_t peers = __ballot(valid);
+        for (int bit = 0; bit < nbits; ++bit) {
+          const bool b = (d >> bit) & 1;
+          const uint64_t m = __ballot(b);
+          peers &= b ? m : ~m;
+        }
+        const int r = lanes_below(peers);
+        const int c = valid ? row[d] : 0;
+        ix[k] = (ix[k] & 0xffffu) | ((uint32_t)(c + r) << 16);
+        wave_lds_sync();
+        if (valid && r == 0) row[d] = c + __popcll(peers);
+        wave_lds_sync();
+      }
+    }
+    __syncthreads();
+    counts_to_offsets<NT, N2, WAVES>(s_cnt, s_scan);
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k)
+      if (vbase + k * 64 + lane < E)
+        ix[k] += (uint32_t)row[(key[k] >> shift) & (uint32_t)(N2 - 1)] << 16;
+    move();
   }
   // s_x[p] = local index of sorted position p (left there by the last move).
   if (bits == 0) {
@@ -663,6 +719,101 @@ __global__ __launch_bounds__(64 * WAVES) void k_window_sort(
   }
 }
 
+// Ascending bitonic sort of one key per lane across the 64 lanes of a wave.
+template <class T>
+__device__ __forceinline__ T wave_bitonic_sort(T key) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const T other = __shfl_xor(key, j, 64);
+      const bool asc = (lane & k) == 0;  // always true in the last merge (k = 64)
+      const bool low = (lane & j) == 0;
+      const T lo = key < other ? key : other;
+      const T hi = key < other ? other : key;
+      key = low == asc ? lo : hi;
+    }
+  }
+  return key;
+}
+
+// Windows of at most 64 entries: one wave per window, four windows per
+// workgroup, no LDS.  Lane q holds local entry q.  A bitonic sort of
+// (column - window minimum, q) gives the sorted order; a sorted entry is new
+// when its column differs from its predecessor's, rank = (new entries up to
+// it) - 1.  A second bitonic sort of (block, q, row, slot) gives the TC-block
+// order directly (inside a block the reference keeps CSR order, which is q
+// order), so lane p then writes TC-block position p.
+__global__ __launch_bounds__(256) void k_window_tiny(
+    const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col, int32_t M,
+    const int32_t *__restrict__ list, int32_t count, const float *__restrict__ values,
+    int32_t *__restrict__ e2r, int32_t *__restrict__ e2c, int32_t *__restrict__ bp,
+    uint8_t *__restrict__ tile, float *__restrict__ tcvalues, int32_t *__restrict__ tcedge,
+    WinStage st, MetcfHeader *h) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= count) return;  // wave-uniform
+  const int w = list[i];
+  const int s = window_begin(rowptr, w, kWinH, M);
+  const int E = window_end(rowptr, w, kWinH, M) - s;
+  if (E <= 0 || E > 64) {  // the class list guarantees 0 < E <= 64
+    if (lane == 0) atomicOr(&h->flags, kBadClass);
+    return;
+  }
+  // lane r (1..15) holds the start of window row r, relative to the window
+  const int rs = (lane >= 1 && lane < kWinH)
+                     ? rowptr[min((int64_t)w * kWinH + lane, (int64_t)M)] - s
+                     : INT32_MAX;
+  const int q = lane;
+  const bool valid = q < E;
+  const int c = col[s + min(q, E - 1)];
+  int row = 0;
+#pragma unroll
+  for (int r = 1; r < kWinH; ++r) row += __shfl(rs, r, 64) <= q ? 1 : 0;
+  if (valid) e2r[s + q] = w * kWinH + row;
+  int cmin = valid ? c : INT32_MAX, cmax = valid ? c : 0;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    cmin = min(cmin, __shfl_xor(cmin, d, 64));
+    cmax = max(cmax, __shfl_xor(cmax, d, 64));
+  }
+  (void)cmax;
+  // sort 1: (column - window minimum, local index); empty lanes last
+  uint64_t k1 = valid ? (((uint64_t)(uint32_t)(c - cmin) << 6) | (uint64_t)q) : ~0ull;
+  k1 = wave_bitonic_sort(k1);
+  const int p = lane;
+  const uint32_t colr = (uint32_t)(k1 >> 6);
+  const int idx = (int)(k1 & 63);
+  const uint32_t prevc = __shfl_up(colr, 1, 64);
+  const bool isnew = p < E && (p == 0 || colr != prevc);
+  const uint64_t m = __ballot(isnew);
+  const int rank = lanes_below(m) + (isnew ? 1 : 0) - 1;
+  const int U = __popcll(m);
+  const int kb = rank / kWinW, cc = rank % kWinW;
+  if (isnew) {
+    st.dcol[s + rank] = (int32_t)(colr + (uint32_t)cmin);
+    if (cc == 0) st.boff[s + kb] = p;
+  }
+  if (lane == 0) {
+    bp[w] = (U + kWinW - 1) / kWinW;
+    st.wU[w] = U;
+  }
+  if (p < E) e2c[s + idx] = rank;
+  // sort 2: (block, local index, row, slot); empty lanes last
+  const int row_s = __shfl(row, idx, 64);  // row of local entry idx
+  uint32_t k2 = p < E ? ((uint32_t)kb << 14) | ((uint32_t)idx << 7) | ((uint32_t)row_s << 3) |
+                            (uint32_t)cc
+                      : ~0u;
+  k2 = wave_bitonic_sort(k2);
+  if (lane < E) {
+    const int i2 = (int)((k2 >> 7) & 63u);
+    tile[s + lane] = (uint8_t)(((k2 >> 3) & 15u) * kWinW + (k2 & 7u));
+    if (tcvalues) tcvalues[s + lane] = values ? values[s + i2] : 1.0f;
+    if (tcedge) tcedge[s + lane] = s + i2;
+  }
+}
+
 // Phase 2 (every window, both paths): a workgroup takes kOutBlocks
 // consecutive TC blocks.  Two waves find the windows of its first and last
 // block (rwo[w] <= b < rwo[w+1]) by cooperative searches; the row-window
@@ -674,7 +825,7 @@ constexpr int kOutBlocks = 512;
 constexpr int kOutWin = 1024;  // row-window offsets kept in LDS
 __global__ __launch_bounds__(256) void k_blocks_out(const int32_t *__restrict__ rowptr, int32_t M,
                                                     int32_t W, int32_t blk_h, int32_t blk_w,
-                                                    int32_t T, WinStage st,
+                                                    int32_t T, int32_t nnz, WinStage st,
                                                     const int32_t *__restrict__ rwo,
                                                     int32_t *__restrict__ off,
                                                     int32_t *__restrict__ rowid,
@@ -685,6 +836,7 @@ __global__ __launch_bounds__(256) void k_blocks_out(const int32_t *__restrict__ 
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int first = blockIdx.x * kOutBlocks;
   if (first >= T) return;  // workgroup-uniform
+  if (first == 0 && tid == 0) off[T] = nnz;
   const int nblk = min(kOutBlocks, T - first);
   if (wave < 2) {
     const int b = wave ? first + nblk - 1 : first;
@@ -736,15 +888,18 @@ __global__ __launch_bounds__(256) void k_blocks_out(const int32_t *__restrict__ 
 // of window w = wmap[w']; key (w', column), value = CSR edge index.  Also
 // writes edgeToRow of those entries.  A workgroup takes 4,096 consecutive
 // compacted entries; two of its threads find the windows of the first and
-// the last one, and each entry searches only between those two.
+// the last one.  When both are the same window (the usual case: these
+// windows hold more than 16,384 entries), its row starts go to LDS once and
+// every entry finds its row there; otherwise each entry searches rowptr.
 constexpr int kBigChunk = 4096;
+constexpr int kBigRowsLds = 256;  // row starts kept in LDS (blk_h <= 255)
 __global__ __launch_bounds__(256) void k_big_keys(const int32_t *rowptr, const int32_t *col,
                                                   int32_t M, int32_t blk_h, int colbits,
-                                                  const MetcfHeader *h, const int32_t *wmap,
-                                                  const int32_t *csb, int32_t *e2r,
-                                                  uint64_t *keys, int32_t *vals) {
+                                                  int32_t n_big, int32_t nnz_big,
+                                                  const int32_t *wmap, const int32_t *csb,
+                                                  int32_t *e2r, uint64_t *keys, int32_t *vals) {
   __shared__ int s_w[2];
-  const int n_big = h->n_big, nnz_big = h->nnz_big;
+  __shared__ int s_rp[kBigRowsLds];
   const int64_t first = (int64_t)blockIdx.x * kBigChunk;
   if (first >= nnz_big) return;  // workgroup-uniform
   const int last = (int)min<int64_t>(first + kBigChunk - 1, nnz_big - 1);
@@ -753,6 +908,31 @@ __global__ __launch_bounds__(256) void k_big_keys(const int32_t *rowptr, const i
     s_w[threadIdx.x] = lower_bound_dev(csb, 0, n_big + 1, (threadIdx.x ? last : (int)first) + 1) - 1;
   __syncthreads();
   const int wa = s_w[0], wb = s_w[1];
+  if (wa == wb && blk_h < kBigRowsLds) {
+    const int w = wmap[wa];
+    const int64_t r0 = (int64_t)w * blk_h;
+    const int nrows = (int)min<int64_t>(blk_h, M - r0);
+    if (threadIdx.x <= nrows) s_rp[threadIdx.x] = rowptr[r0 + threadIdx.x];
+    __syncthreads();
+    const int s = s_rp[0];
+    const int delta = s - csb[wa];
+    const uint64_t wkey = (uint64_t)wa << colbits;
+#pragma unroll 4
+    for (int j = (int)first + threadIdx.x; j <= last; j += blockDim.x) {
+      const int e = j + delta;
+      // row: the last r in [0, nrows) with s_rp[r] <= e
+      int lo = 0, hi = nrows;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (s_rp[mid] <= e) lo = mid;
+        else hi = mid;
+      }
+      e2r[e] = (int)r0 + lo;
+      keys[j] = wkey | (uint32_t)col[e];
+      vals[j] = e;
+    }
+    return;
+  }
   for (int j = (int)first + threadIdx.x; j <= last; j += blockDim.x) {
     const int wp = wa == wb ? wa : lower_bound_dev(csb, wa, wb + 1, j + 1) - 1;
     const int w = wmap[wp];
@@ -783,35 +963,36 @@ struct BigView {
   }
 };
 
-__device__ __forceinline__ BigView big_view(const MetcfHeader *h, const uint64_t *keys0,
-                                            const uint64_t *keys1, const int32_t *vals0,
-                                            const int32_t *vals1, const int32_t *g,
-                                            const int32_t *csb, const int32_t *wmap) {
+// The sorted global-path pairs (key, CSR edge index), their unique-column
+// scan and window maps, as the kernels after the sort read them.
+BigView big_view(const uint64_t *keys, const int32_t *vals, const int32_t *g, const int32_t *csb,
+                 const int32_t *wmap, int colbits) {
   BigView bv;
-  bv.keys = h->sorted_in_1 ? keys1 : keys0;
-  bv.vals = h->sorted_in_1 ? vals1 : vals0;
+  bv.keys = keys;
+  bv.vals = vals;
   bv.g = g;
   bv.csb = csb;
   bv.wmap = wmap;
-  bv.colbits = h->colbits;
+  bv.colbits = colbits;
   return bv;
 }
 
 // blockPartition of the global-path windows: ceil(|S_w| / blk_w)
-// (DTCSpMM_kernel.cu:215-223); also |S_w| (WinStage) and the block count per
-// compacted window (prefix-summed into cbb).
-__global__ __launch_bounds__(256) void k_big_counts(const MetcfHeader *h, const int32_t *g,
-                                                    const int32_t *csb, const int32_t *wmap,
-                                                    int32_t blk_w, int32_t *bp, int32_t *wU,
-                                                    int32_t *nbc) {
-  const int wp = blockIdx.x * blockDim.x + threadIdx.x;
-  if (wp >= h->n_big) return;
-  const int c0 = csb[wp], c1 = csb[wp + 1];
-  const int U = g[c1 - 1] - (c0 > 0 ? g[c0 - 1] : 0);
-  const int nb = (U - 1 + blk_w) / blk_w;
-  bp[wmap[wp]] = nb;
-  wU[wmap[wp]] = U;
-  nbc[wp] = nb;
+// (DTCSpMM_kernel.cu:215-223), |S_w| (WinStage), and cbb[k] = blocks of the
+// windows listed before k (one workgroup).
+__global__ __launch_bounds__(1024) void k_big_counts(BigView bv, int32_t n_big, int32_t blk_w,
+                                                     int32_t *bp, int32_t *wU, int32_t *cbb) {
+  __shared__ int s_tmp[16];
+  wg1024_exclusive_scan(
+      [&](int wp) {
+        const int c0 = bv.csb[wp], c1 = bv.csb[wp + 1];
+        const int U = bv.g[c1 - 1] - (c0 > 0 ? bv.g[c0 - 1] : 0);
+        const int nb = (U - 1 + blk_w) / blk_w;
+        bp[bv.wmap[wp]] = nb;
+        wU[bv.wmap[wp]] = U;
+        return nb;
+      },
+      [&](int wp, int x) { cbb[wp] = x; }, n_big, s_tmp);
 }
 
 // Per sorted entry j of the global path: edgeToColumn[e] = rank of col(e) in
@@ -819,14 +1000,15 @@ __global__ __launch_bounds__(256) void k_big_counts(const MetcfHeader *h, const 
 // column stores the column (WinStage dcol), the first of each block its
 // window-relative position (WinStage boff) and its compacted position
 // (bstart, for the tile pass).
-__global__ __launch_bounds__(256) void k_big_stage(
-    const int32_t *rowptr, int32_t M, int32_t blk_h, int32_t blk_w, const MetcfHeader *h,
-    const uint64_t *keys0, const uint64_t *keys1, const int32_t *vals0, const int32_t *vals1,
-    const int32_t *g, const int32_t *csb, const int32_t *wmap, const int32_t *cbb,
-    int32_t *e2c, WinStage st, int32_t *bstart) {
+__global__ __launch_bounds__(256) void k_big_stage(const int32_t *rowptr, int32_t M, int32_t blk_h,
+                                                   int32_t blk_w, int32_t nnz_big, BigView bv,
+                                                   const int32_t *cbb, int32_t *e2c, WinStage st,
+                                                   int32_t *bstart) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= h->nnz_big) return;
-  const BigView bv = big_view(h, keys0, keys1, vals0, vals1, g, csb, wmap);
+  if (j >= nnz_big) return;
+  const int32_t *g = bv.g;
+  const int32_t *csb = bv.csb;
+  const int32_t *wmap = bv.wmap;
   const int wp = bv.window(j);
   const int r = g[j] - bv.base(wp) - 1;
   e2c[bv.vals[j]] = r;
@@ -850,13 +1032,13 @@ __global__ __launch_bounds__(256) void k_big_stage(
 // entries).
 __global__ __launch_bounds__(256) void k_big_tile_ids(
     const int32_t *rowptr, const int32_t *e2r, int32_t M, int32_t blk_h, int32_t blk_w,
-    const MetcfHeader *h, const uint64_t *keys0, const uint64_t *keys1, const int32_t *vals0,
-    const int32_t *vals1, const int32_t *g, const int32_t *csb, const int32_t *wmap,
-    const int32_t *cbb, const int32_t *bstart, const float *values, uint8_t *tile,
-    float *tcvalues, int32_t *tcedge) {
+    int32_t nnz_big, BigView bv, const int32_t *cbb, const int32_t *bstart, const float *values,
+    uint8_t *tile, float *tcvalues, int32_t *tcedge) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= h->nnz_big) return;
-  const BigView bv = big_view(h, keys0, keys1, vals0, vals1, g, csb, wmap);
+  if (j >= nnz_big) return;
+  const int32_t *g = bv.g;
+  const int32_t *csb = bv.csb;
+  const int32_t *wmap = bv.wmap;
   const int wp = bv.window(j);
   const int w = wmap[wp];
   const int c0 = csb[wp], c1 = csb[wp + 1];
@@ -878,26 +1060,6 @@ __global__ __launch_bounds__(256) void k_big_tile_ids(
   tile[dst] = (uint8_t)(row_local * (unsigned)blk_w + (unsigned)c);
   if (tcvalues) tcvalues[dst] = values ? values[myv] : 1.0f;
   if (tcedge) tcedge[dst] = myv;
-}
-
-// Sum of blockPartition over non-empty windows (the row-window offsets must
-// end at it; a non-zero entry of an empty window means the caller did not
-// zero-fill blockPartition).
-__global__ __launch_bounds__(256) void k_sum_blocks(const int32_t *rowptr, int32_t M, int32_t W,
-                                                    int32_t blk_h, const int32_t *bp,
-                                                    int32_t *rwo, MetcfHeader *h) {
-  __shared__ int smem[4];
-  const int w = blockIdx.x * blockDim.x + threadIdx.x;
-  int nb = 0;
-  if (w < W && window_end(rowptr, w, blk_h, M) > window_begin(rowptr, w, blk_h, M)) nb = bp[w];
-  int total;
-  block256_inclusive_sum(nb, smem, total);
-  if (threadIdx.x == 0 && total) atomicAdd(&h->block_count, total);
-  if (w == 0) rwo[0] = 0;
-}
-
-__global__ void k_copy_last(const int32_t *rwo, int32_t W, MetcfHeader *h) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) h->rwo_last = rwo[W];
 }
 
 __global__ void k_set_i32(int32_t *p, int32_t v) {
@@ -994,16 +1156,9 @@ dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *no
   }
   if (W > 0) {
     k_classify<<<grid_for(W, 256), 256, 0, stream>>>(node_pointer, num_nodes, (int32_t)W, blk_h,
-                                                      mode, ws.lists, ws.widx, ws.wcs, ws.wcls,
-                                                      ws.hdr);
+                                                      mode, block_partition, ws.lists,
+                                                      row_window_offset, ws.hdr);
     DTC_LAUNCH_CHECK("classify");
-    st = scan_i32(ws.widx, ws.widx, W, true, ws.tmp, stream);
-    if (st != DTC_SUCCESS) return st;
-    st = scan_i32(ws.wcs, ws.wcs, W, true, ws.tmp, stream);
-    if (st != DTC_SUCCESS) return st;
-    k_big_map<<<grid_for(W, 256), 256, 0, stream>>>((int32_t)W, ws.widx, ws.wcs, ws.wmap, ws.csb,
-                                                     ws.hdr);
-    DTC_LAUNCH_CHECK("big_map");
   }
   MetcfHeader host{};
   DTC_HIP_CHECK(hipMemcpyAsync(&host, ws.hdr, 64, hipMemcpyDeviceToHost, stream));
@@ -1014,6 +1169,9 @@ dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *no
     if (host.flags & kBadLast) why += " nodePointer[num_nodes] != number of edges;";
     if (host.flags & kBadOrder) why += " nodePointer decreases;";
     if (host.flags & kBadColumn) why += " negative column index;";
+    if (why.empty() && (host.flags & kBadPartition))
+      return fail(DTC_ERROR_INCONSISTENT,
+                  "blockPartition must be zero-filled: an empty row window has a non-zero entry");
     return fail(DTC_ERROR_INVALID_CSR, "invalid CSR:" + why);
   }
   if (W == 0) {
@@ -1023,17 +1181,34 @@ dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *no
     return DTC_SUCCESS;
   }
   const int colbits = std::max(1, bits_for(host.max_col));
+  const int n_big = host.cls_count[kClasses];
+  const int nnz_big = host.nnz_big;
+  const int32_t *wmap = ws.lists + (int64_t)kClasses * W;
+  if (n_big > 0) {
+    k_big_offsets<<<1, 1024, 0, stream>>>(node_pointer, num_nodes, blk_h, wmap, n_big, ws.csb);
+    DTC_LAUNCH_CHECK("big_offsets");
+  }
 
   // 2. LDS path, largest class first: every output of the window sized by
   //    its entries, and its block data for phase 2.
   for (int c = kClasses - 1; c >= 0 && st == DTC_SUCCESS; --c) {
     const int n = host.cls_count[c];
+    if (c == 0) {
+      if (n > 0) {
+        k_window_tiny<<<grid_for(n, 4), 256, 0, stream>>>(
+            node_pointer, edge_list, num_nodes, ws.lists, n, values, edge_to_row,
+            edge_to_column, block_partition, tcblocktile_id, tcblock_values, tcblock_edge,
+            win_stage(ws), ws.hdr);
+        DTC_LAUNCH_CHECK("window_tiny");
+      }
+      continue;
+    }
     switch (c) {
+      case 5: st = launch_window_sort<5>(node_pointer, edge_list, num_nodes, n, ws, W, values, edge_to_row, edge_to_column, block_partition, tcblocktile_id, tcblock_values, tcblock_edge, stream); break;
       case 4: st = launch_window_sort<4>(node_pointer, edge_list, num_nodes, n, ws, W, values, edge_to_row, edge_to_column, block_partition, tcblocktile_id, tcblock_values, tcblock_edge, stream); break;
       case 3: st = launch_window_sort<3>(node_pointer, edge_list, num_nodes, n, ws, W, values, edge_to_row, edge_to_column, block_partition, tcblocktile_id, tcblock_values, tcblock_edge, stream); break;
       case 2: st = launch_window_sort<2>(node_pointer, edge_list, num_nodes, n, ws, W, values, edge_to_row, edge_to_column, block_partition, tcblocktile_id, tcblock_values, tcblock_edge, stream); break;
-      case 1: st = launch_window_sort<1>(node_pointer, edge_list, num_nodes, n, ws, W, values, edge_to_row, edge_to_column, block_partition, tcblocktile_id, tcblock_values, tcblock_edge, stream); break;
-      default: st = launch_window_sort<0>(node_pointer, edge_list, num_nodes, n, ws, W, values, edge_to_row, edge_to_column, block_partition, tcblocktile_id, tcblock_values, tcblock_edge, stream); break;
+      default: st = launch_window_sort<1>(node_pointer, edge_list, num_nodes, n, ws, W, values, edge_to_row, edge_to_column, block_partition, tcblocktile_id, tcblock_values, tcblock_edge, stream); break;
     }
   }
   if (st != DTC_SUCCESS) return st;
@@ -1041,61 +1216,45 @@ dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *no
   // 3. Global path: keys (compacted window, column), value = CSR edge index;
   //    stable sort; unique-column numbering; blockPartition, edgeToColumn,
   //    block data, tile ids.
-  if (host.nnz_big > 0) {
-    const int64_t nb = host.nnz_big;
-    k_big_keys<<<grid_for(nb, kBigChunk), 256, 0, stream>>>(node_pointer, edge_list, num_nodes,
-                                                            blk_h, colbits, ws.hdr, ws.wmap,
-                                                            ws.csb, edge_to_row, ws.keys0,
-                                                            ws.vals0);
+  if (nnz_big > 0) {
+    k_big_keys<<<grid_for(nnz_big, kBigChunk), 256, 0, stream>>>(
+        node_pointer, edge_list, num_nodes, blk_h, colbits, n_big, nnz_big, wmap, ws.csb,
+        edge_to_row, ws.keys0, ws.vals0);
     DTC_LAUNCH_CHECK("big_keys");
     bool in_1 = false;
-    const int key_bits = colbits + bits_for(host.n_big - 1);
-    st = radix_sort_pairs(ws.keys0, ws.vals0, ws.keys1, ws.vals1, nb, key_bits, &in_1, ws.tmp,
+    const int key_bits = colbits + bits_for(n_big - 1);
+    st = radix_sort_pairs(ws.keys0, ws.vals0, ws.keys1, ws.vals1, nnz_big, key_bits, &in_1, ws.tmp,
                           stream);
     if (st != DTC_SUCCESS) return st;
-    st = scan_key_changes(in_1 ? ws.keys1 : ws.keys0, ws.gscan, nb, ws.tmp, stream);
+    st = scan_key_changes(in_1 ? ws.keys1 : ws.keys0, ws.gscan, nnz_big, ws.tmp, stream);
     if (st != DTC_SUCCESS) return st;
-    k_header_set<<<1, 64, 0, stream>>>(ws.hdr, colbits, in_1 ? 1 : 0);
-    DTC_LAUNCH_CHECK("header_set");
-    k_big_counts<<<grid_for(host.n_big, 256), 256, 0, stream>>>(
-        ws.hdr, ws.gscan, ws.csb, ws.wmap, blk_w, block_partition, ws.wU, ws.cbb);
+    const BigView bv = big_view(in_1 ? ws.keys1 : ws.keys0, in_1 ? ws.vals1 : ws.vals0, ws.gscan,
+                                ws.csb, wmap, colbits);
+    k_big_counts<<<1, 1024, 0, stream>>>(bv, n_big, blk_w, block_partition, ws.wU, ws.cbb);
     DTC_LAUNCH_CHECK("big_counts");
-    st = scan_i32(ws.cbb, ws.cbb, host.n_big, false, ws.tmp, stream);
-    if (st != DTC_SUCCESS) return st;
-    const unsigned grid = grid_for(nb, 256);
-    k_big_stage<<<grid, 256, 0, stream>>>(node_pointer, num_nodes, blk_h, blk_w, ws.hdr, ws.keys0,
-                                          ws.keys1, ws.vals0, ws.vals1, ws.gscan, ws.csb,
-                                          ws.wmap, ws.cbb, edge_to_column, win_stage(ws),
-                                          ws.bstart);
+    const unsigned grid = grid_for(nnz_big, 256);
+    k_big_stage<<<grid, 256, 0, stream>>>(node_pointer, num_nodes, blk_h, blk_w, nnz_big, bv,
+                                          ws.cbb, edge_to_column, win_stage(ws), ws.bstart);
     DTC_LAUNCH_CHECK("big_stage");
     k_big_tile_ids<<<grid, 256, 0, stream>>>(node_pointer, edge_to_row, num_nodes, blk_h, blk_w,
-                                             ws.hdr, ws.keys0, ws.keys1, ws.vals0, ws.vals1,
-                                             ws.gscan, ws.csb, ws.wmap, ws.cbb, ws.bstart,
-                                             values, tcblocktile_id, tcblock_values,
-                                             tcblock_edge);
+                                             nnz_big, bv, ws.cbb, ws.bstart, values,
+                                             tcblocktile_id, tcblock_values, tcblock_edge);
     DTC_LAUNCH_CHECK("big_tile_ids");
   }
 
-  // 4. Block count of the non-empty windows, window offsets.
-  k_sum_blocks<<<grid_for(W, 256), 256, 0, stream>>>(node_pointer, num_nodes, (int32_t)W, blk_h,
-                                                      block_partition, row_window_offset, ws.hdr);
-  DTC_LAUNCH_CHECK("sum_blocks");
+  // 4. Window offsets (row_window_offset[0] was written by k_classify).
   st = scan_i32(block_partition, row_window_offset + 1, W, true, ws.tmp, stream);
   if (st != DTC_SUCCESS) return st;
-  k_copy_last<<<1, 64, 0, stream>>>(row_window_offset, (int32_t)W, ws.hdr);
-  DTC_LAUNCH_CHECK("copy_last");
 
   // 5. Block count (second host synchronisation).
+  int32_t T = 0;
   DTC_HIP_CHECK(hipMemcpyAsync(&host, ws.hdr, 64, hipMemcpyDeviceToHost, stream));
+  DTC_HIP_CHECK(hipMemcpyAsync(&T, row_window_offset + W, sizeof(int32_t), hipMemcpyDeviceToHost,
+                               stream));
   DTC_HIP_CHECK(hipStreamSynchronize(stream));
   if (host.flags & kBadClass)
     return fail(DTC_ERROR_INCONSISTENT, "internal error: a window exceeded its class capacity");
-  if (host.block_count != host.rwo_last)
-    return fail(DTC_ERROR_INCONSISTENT,
-                "blockPartition must be zero-filled: row-window offsets end at " +
-                    std::to_string(host.rwo_last) + " but the windows hold " +
-                    std::to_string(host.block_count) + " blocks");
-  *block_count = host.block_count;
+  *block_count = T;
   return DTC_SUCCESS;
 }
 
@@ -1121,11 +1280,13 @@ dtc_status_t dtc_metcf_build_blocks(const int32_t *node_pointer,
   MetcfWs ws = carve(workspace, nnz, W);
   if (!workspace || workspace_bytes < ws.bytes)
     return fail(DTC_ERROR_WORKSPACE, "ME-TCF workspace needs " + std::to_string(ws.bytes) + " bytes");
-  k_set_i32<<<1, 64, 0, stream>>>(tcblock_offset + block_count, nnz);
-  DTC_LAUNCH_CHECK("set");
-  if (block_count == 0) return DTC_SUCCESS;
+  if (block_count == 0) {
+    k_set_i32<<<1, 64, 0, stream>>>(tcblock_offset, nnz);
+    DTC_LAUNCH_CHECK("set");
+    return DTC_SUCCESS;
+  }
   k_blocks_out<<<grid_for(block_count, kOutBlocks), 256, 0, stream>>>(
-      node_pointer, num_nodes, (int32_t)W, blk_h, blk_w, block_count, win_stage(ws),
+      node_pointer, num_nodes, (int32_t)W, blk_h, blk_w, block_count, nnz, win_stage(ws),
       row_window_offset, tcblock_offset, tcblock_rowid, sparse_atox);
   DTC_LAUNCH_CHECK("blocks_out");
   return DTC_SUCCESS;

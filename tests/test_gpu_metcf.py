@@ -3,9 +3,9 @@
 Most tests run under every window routing of the build (DTC_METCF_PATH):
 "auto" (by entry count: one LDS workgroup per window up to 16,384 entries,
 the device-wide radix sort beyond), "global" (every window through the
-radix sort) and "lds0" ... "lds4" (every window that fits the 1-, 2-, 4-,
-8- or 16-wave workgroup shape through it, the rest through the radix
-sort)."""
+radix sort) and "lds0" ... "lds5" (every window that fits the one-wave
+bitonic kernel for at most 64 entries, or the 1-, 2-, 4-, 8- or 16-wave
+LDS radix-sort shape, through it; the rest through the radix sort)."""
 import os
 
 import numpy as np
@@ -24,7 +24,7 @@ def ext():
     return load_extension()
 
 
-PATHS = ["auto", "global", "lds0", "lds1", "lds2", "lds3", "lds4"]
+PATHS = ["auto", "global", "lds0", "lds1", "lds2", "lds3", "lds4", "lds5"]
 
 
 @pytest.fixture(params=PATHS)
