@@ -35,6 +35,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "dtc_internal.h"
@@ -90,6 +91,11 @@ struct MetcfWs {
   int32_t *bstart;          // global path: first compacted entry of each of its blocks
   int32_t *cbb;             // global path: first block of each compacted window [n_big + 1]
   int32_t *csb;             // compacted window -> first compacted entry [n_big + 1]
+  int32_t *tb;              // global path: first sort tile of each compacted window [n_big + 1]
+  int32_t *tw;              // global path: compacted window of each sort tile
+  // 32-bit views of keys0 / keys1 for the segmented global-path sort
+  uint32_t *k32_0, *k32_1;
+  int32_t *wpos;            // global path: compacted window of each compacted position
   void *tmp;                // scan / sort scratch
   size_t bytes;
 };
@@ -112,7 +118,12 @@ MetcfWs carve(void *ws, int64_t nnz, int64_t W) {
   w.bstart = c.take<int32_t>(n);
   w.cbb = c.take<int32_t>(nw + 1);
   w.csb = c.take<int32_t>(nw + 1);
-  const size_t tmp = std::max({radix_sort_workspace_bytes(nnz), scan_workspace_bytes(nnz),
+  w.tb = c.take<int32_t>(nw + 1);
+  w.tw = c.take<int32_t>((size_t)(ceil_div(std::max<int64_t>(nnz, 1), seg_radix_tile_size()) + nw));
+  w.k32_0 = reinterpret_cast<uint32_t *>(w.keys0);
+  w.k32_1 = reinterpret_cast<uint32_t *>(w.keys1);
+  w.wpos = c.take<int32_t>(n);
+  const size_t tmp = std::max({seg_radix_sort_workspace_bytes(nnz, W), scan_workspace_bytes(nnz),
                                scan_workspace_bytes(W)});
   w.tmp = c.take<char>(tmp);
   w.bytes = c.used();
@@ -316,17 +327,55 @@ __device__ __forceinline__ int wg1024_exclusive_scan(Load load, Store store, int
 
 // First compacted entry of each global-path window: csb[k] = entries of the
 // windows listed before k; csb[n_big] = all of them.
+// Also cuts each window into sort tiles of `tile` entries: tb[k] = tiles of
+// the windows listed before k, tb[n_big] = all of them, tw[t] = window of
+// tile t.
 __global__ __launch_bounds__(1024) void k_big_offsets(const int32_t *rowptr, int32_t M,
                                                       int32_t blk_h, const int32_t *wmap,
-                                                      int32_t n_big, int32_t *csb) {
+                                                      int32_t n_big, int32_t tile, int32_t *csb,
+                                                      int32_t *tb, int32_t *tw) {
   __shared__ int s_tmp[16];
-  const int total = wg1024_exclusive_scan(
-      [&](int i) {
-        const int w = wmap[i];
-        return window_end(rowptr, w, blk_h, M) - window_begin(rowptr, w, blk_h, M);
-      },
-      [&](int i, int x) { csb[i] = x; }, n_big, s_tmp);
-  if (threadIdx.x == 0) csb[n_big] = total;
+  auto entries = [&](int i) {
+    const int w = wmap[i];
+    return window_end(rowptr, w, blk_h, M) - window_begin(rowptr, w, blk_h, M);
+  };
+  const int total = wg1024_exclusive_scan(entries, [&](int i, int x) { csb[i] = x; }, n_big, s_tmp);
+  const int tiles = wg1024_exclusive_scan([&](int i) { return (entries(i) + tile - 1) / tile; },
+                                          [&](int i, int x) { tb[i] = x; }, n_big, s_tmp);
+  if (threadIdx.x == 0) {
+    csb[n_big] = total;
+    tb[n_big] = tiles;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n_big; i += blockDim.x)
+    for (int t = tb[i]; t < tb[i + 1]; ++t) tw[t] = i;
+}
+
+// Many global-path windows: the same offsets with one thread per window
+// and device-wide scans (csb and tb hold the counts, with a trailing 0, until
+// scanned), then the tile map.
+constexpr int kOneGroupWindows = 4096;  // up to this many: one-workgroup kernels
+__global__ __launch_bounds__(256) void k_big_sizes(const int32_t *rowptr, int32_t M,
+                                                   int32_t blk_h, const int32_t *wmap,
+                                                   int32_t n_big, int32_t tile, int32_t *csb,
+                                                   int32_t *tb) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n_big) {
+    const int w = wmap[k];
+    const int E = window_end(rowptr, w, blk_h, M) - window_begin(rowptr, w, blk_h, M);
+    csb[k] = E;
+    tb[k] = (E + tile - 1) / tile;
+  } else if (k == n_big) {
+    csb[k] = 0;
+    tb[k] = 0;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_big_tilemap(const int32_t *tb, int32_t n_big,
+                                                     int32_t *tw) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n_big)
+    for (int t = tb[k]; t < tb[k + 1]; ++t) tw[t] = k;
 }
 
 // ------------------------------------------------------------- LDS path
@@ -885,7 +934,11 @@ __global__ __launch_bounds__(256) void k_blocks_out(const int32_t *__restrict__ 
 
 // Keys of the global-path windows: compacted entry j of window w'
 // (entries [csb[w'], csb[w'+1])) is CSR entry rowptr[w*blk_h] + j - csb[w']
-// of window w = wmap[w']; key (w', column), value = CSR edge index.  Also
+// of window w = wmap[w']; value = CSR edge index; wpos[j] = w'.  Key, for
+// the segmented sort (32 bits, each window sorted within its own range):
+// column with bit 31 = parity of w', so that neighbouring windows never share
+// a key; for the sort of the whole range (64 bits): (w', column).  Either
+// way the sorted order keeps every entry inside its window's range.  Also
 // writes edgeToRow of those entries.  A workgroup takes 4,096 consecutive
 // compacted entries; two of its threads find the windows of the first and
 // the last one.  When both are the same window (the usual case: these
@@ -893,11 +946,19 @@ __global__ __launch_bounds__(256) void k_blocks_out(const int32_t *__restrict__ 
 // every entry finds its row there; otherwise each entry searches rowptr.
 constexpr int kBigChunk = 4096;
 constexpr int kBigRowsLds = 256;  // row starts kept in LDS (blk_h <= 255)
+template <class Key>
+__device__ __forceinline__ Key big_key(int wp, int c, int colbits) {
+  if (sizeof(Key) == 4) return (Key)(((uint32_t)(wp & 1) << 31) | (uint32_t)c);
+  return (Key)(((uint64_t)wp << colbits) | (uint32_t)c);
+}
+
+template <class Key>
 __global__ __launch_bounds__(256) void k_big_keys(const int32_t *rowptr, const int32_t *col,
                                                   int32_t M, int32_t blk_h, int colbits,
                                                   int32_t n_big, int32_t nnz_big,
                                                   const int32_t *wmap, const int32_t *csb,
-                                                  int32_t *e2r, uint64_t *keys, int32_t *vals) {
+                                                  int32_t *e2r, Key *keys, int32_t *vals,
+                                                  int32_t *wpos) {
   __shared__ int s_w[2];
   __shared__ int s_rp[kBigRowsLds];
   const int64_t first = (int64_t)blockIdx.x * kBigChunk;
@@ -916,7 +977,6 @@ __global__ __launch_bounds__(256) void k_big_keys(const int32_t *rowptr, const i
     __syncthreads();
     const int s = s_rp[0];
     const int delta = s - csb[wa];
-    const uint64_t wkey = (uint64_t)wa << colbits;
 #pragma unroll 4
     for (int j = (int)first + threadIdx.x; j <= last; j += blockDim.x) {
       const int e = j + delta;
@@ -928,8 +988,9 @@ __global__ __launch_bounds__(256) void k_big_keys(const int32_t *rowptr, const i
         else hi = mid;
       }
       e2r[e] = (int)r0 + lo;
-      keys[j] = wkey | (uint32_t)col[e];
+      keys[j] = big_key<Key>(wa, col[e], colbits);
       vals[j] = e;
+      wpos[j] = wa;
     }
     return;
   }
@@ -943,19 +1004,26 @@ __global__ __launch_bounds__(256) void k_big_keys(const int32_t *rowptr, const i
     const int rend = (int)min<int64_t>(r0 + blk_h, M);
     const int row = lower_bound_dev(rowptr, (int)r0, rend, e + 1) - 1;
     e2r[e] = row;
-    keys[j] = ((uint64_t)wp << colbits) | (uint32_t)col[e];
+    keys[j] = big_key<Key>(wp, col[e], colbits);
     vals[j] = e;
+    wpos[j] = wp;
   }
 }
 
 struct BigView {
-  const uint64_t *keys;
+  const uint32_t *keys32;  // segmented sort
+  const uint64_t *keys64;  // sort of the whole range
   const int32_t *vals;
   const int32_t *g;
   const int32_t *csb;
   const int32_t *wmap;
+  const int32_t *wpos;
   int colbits;
-  __device__ __forceinline__ int window(int j) const { return (int)(keys[j] >> colbits); }
+  __device__ __forceinline__ int window(int j) const { return wpos[j]; }
+  __device__ __forceinline__ int column(int j) const {
+    return keys32 ? (int)(keys32[j] & 0x7fffffffu)
+                  : (int)(keys64[j] & ((1ull << colbits) - 1));
+  }
   // unique-column count before compacted window wp
   __device__ __forceinline__ int base(int wp) const {
     const int c = csb[wp];
@@ -965,15 +1033,17 @@ struct BigView {
 
 // The sorted global-path pairs (key, CSR edge index), their unique-column
 // scan and window maps, as the kernels after the sort read them.
-BigView big_view(const uint64_t *keys, const int32_t *vals, const int32_t *g, const int32_t *csb,
-                 const int32_t *wmap, int colbits) {
+BigView big_view(const uint32_t *keys32, const uint64_t *keys64, int colbits, const int32_t *vals,
+                 const int32_t *g, const int32_t *csb, const int32_t *wmap, const int32_t *wpos) {
   BigView bv;
-  bv.keys = keys;
+  bv.keys32 = keys32;
+  bv.keys64 = keys64;
+  bv.colbits = colbits;
   bv.vals = vals;
   bv.g = g;
   bv.csb = csb;
   bv.wmap = wmap;
-  bv.colbits = colbits;
+  bv.wpos = wpos;
   return bv;
 }
 
@@ -993,6 +1063,24 @@ __global__ __launch_bounds__(1024) void k_big_counts(BigView bv, int32_t n_big, 
         return nb;
       },
       [&](int wp, int x) { cbb[wp] = x; }, n_big, s_tmp);
+}
+
+// The same with one thread per window (cbb then holds the block counts, with
+// a trailing 0, until scanned).
+__global__ __launch_bounds__(256) void k_big_counts_multi(BigView bv, int32_t n_big, int32_t blk_w,
+                                                          int32_t *bp, int32_t *wU,
+                                                          int32_t *cbb) {
+  const int wp = blockIdx.x * blockDim.x + threadIdx.x;
+  if (wp < n_big) {
+    const int c0 = bv.csb[wp], c1 = bv.csb[wp + 1];
+    const int U = bv.g[c1 - 1] - (c0 > 0 ? bv.g[c0 - 1] : 0);
+    const int nb = (U - 1 + blk_w) / blk_w;
+    bp[bv.wmap[wp]] = nb;
+    wU[bv.wmap[wp]] = U;
+    cbb[wp] = nb;
+  } else if (wp == n_big) {
+    cbb[wp] = 0;
+  }
 }
 
 // Per sorted entry j of the global path: edgeToColumn[e] = rank of col(e) in
@@ -1015,7 +1103,7 @@ __global__ __launch_bounds__(256) void k_big_stage(const int32_t *rowptr, int32_
   if (j > 0 && g[j - 1] == g[j]) return;
   const int w = wmap[wp];
   const int s = window_begin(rowptr, w, blk_h, M);
-  st.dcol[s + r] = (int)(bv.keys[j] & ((1ull << bv.colbits) - 1));
+  st.dcol[s + r] = bv.column(j);
   if (r % blk_w == 0) {
     const int kb = r / blk_w;
     st.boff[s + kb] = j - csb[wp];
@@ -1086,6 +1174,35 @@ WinStage win_stage(const MetcfWs &ws) {
   st.dcol = ws.dcol;
   st.wU = ws.wU;
   return st;
+}
+
+// The global path runs on a library-owned stream of the current device
+// (default priority, created once), forked from and joined back to the
+// caller's stream with events, so that it overlaps the LDS path: the two
+// write disjoint windows.  `mu` is held from the fork to the join of one
+// build so that concurrent builds do not interleave their event records.
+struct SideStream {
+  hipStream_t s = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+  std::mutex mu;
+};
+
+dtc_status_t side_stream(SideStream **out) {
+  static std::mutex reg;
+  static SideStream *per_dev[64] = {};
+  int dev = 0;
+  DTC_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return fail(DTC_ERROR_UNSUPPORTED, "device index above 63");
+  std::lock_guard<std::mutex> g(reg);
+  if (!per_dev[dev]) {
+    SideStream *x = new SideStream;
+    DTC_HIP_CHECK(hipStreamCreateWithFlags(&x->s, hipStreamNonBlocking));
+    DTC_HIP_CHECK(hipEventCreateWithFlags(&x->fork, hipEventDisableTiming));
+    DTC_HIP_CHECK(hipEventCreateWithFlags(&x->join, hipEventDisableTiming));
+    per_dev[dev] = x;
+  }
+  *out = per_dev[dev];
+  return DTC_SUCCESS;
 }
 
 template <int C>
@@ -1184,12 +1301,24 @@ dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *no
   const int n_big = host.cls_count[kClasses];
   const int nnz_big = host.nnz_big;
   const int32_t *wmap = ws.lists + (int64_t)kClasses * W;
-  if (n_big > 0) {
-    k_big_offsets<<<1, 1024, 0, stream>>>(node_pointer, num_nodes, blk_h, wmap, n_big, ws.csb);
-    DTC_LAUNCH_CHECK("big_offsets");
-  }
 
-  // 2. LDS path, largest class first: every output of the window sized by
+  // 2. Fork: the global path goes to the side stream when the LDS path also
+  //    has work.  The LDS launches are issued first (few, long) so that they
+  //    start while the host issues the global path's many short ones.
+  int n_lds = 0;
+  for (int c = 0; c < kClasses; ++c) n_lds += host.cls_count[c];
+  SideStream *side = nullptr;
+  std::unique_lock<std::mutex> side_lock;
+  hipStream_t gs = stream;
+  if (nnz_big > 0 && n_lds > 0) {
+    st = side_stream(&side);
+    if (st != DTC_SUCCESS) return st;
+    side_lock = std::unique_lock<std::mutex>(side->mu);
+    gs = side->s;
+    DTC_HIP_CHECK(hipEventRecord(side->fork, stream));
+    DTC_HIP_CHECK(hipStreamWaitEvent(gs, side->fork, 0));
+  }
+  // 3. LDS path, largest class first: every output of the window sized by
   //    its entries, and its block data for phase 2.
   for (int c = kClasses - 1; c >= 0 && st == DTC_SUCCESS; --c) {
     const int n = host.cls_count[c];
@@ -1212,41 +1341,96 @@ dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *no
     }
   }
   if (st != DTC_SUCCESS) return st;
+  // 4. Global path: window offsets and sort tiles; keys, sort, unique-column
+  //    numbering; blockPartition, edgeToColumn, block data, tile ids; then
+  //    join.
+  if (n_big > 0 && n_big <= kOneGroupWindows) {
+    k_big_offsets<<<1, 1024, 0, gs>>>(node_pointer, num_nodes, blk_h, wmap, n_big,
+                                          (int32_t)seg_radix_tile_size(), ws.csb, ws.tb, ws.tw);
+    DTC_LAUNCH_CHECK("big_offsets");
+  } else if (n_big > 0) {
+    k_big_sizes<<<grid_for(n_big + 1, 256), 256, 0, gs>>>(
+        node_pointer, num_nodes, blk_h, wmap, n_big, (int32_t)seg_radix_tile_size(), ws.csb,
+        ws.tb);
+    DTC_LAUNCH_CHECK("big_sizes");
+    st = scan_i32(ws.csb, ws.csb, n_big + 1, false, ws.tmp, gs);
+    if (st != DTC_SUCCESS) return st;
+    st = scan_i32(ws.tb, ws.tb, n_big + 1, false, ws.tmp, gs);
+    if (st != DTC_SUCCESS) return st;
+    k_big_tilemap<<<grid_for(n_big, 256), 256, 0, gs>>>(ws.tb, n_big, ws.tw);
+    DTC_LAUNCH_CHECK("big_tilemap");
+  }
 
-  // 3. Global path: keys (compacted window, column), value = CSR edge index;
-  //    stable sort; unique-column numbering; blockPartition, edgeToColumn,
-  //    block data, tile ids.
   if (nnz_big > 0) {
-    k_big_keys<<<grid_for(nnz_big, kBigChunk), 256, 0, stream>>>(
-        node_pointer, edge_list, num_nodes, blk_h, colbits, n_big, nnz_big, wmap, ws.csb,
-        edge_to_row, ws.keys0, ws.vals0);
-    DTC_LAUNCH_CHECK("big_keys");
+    // Windows averaging at least one sort tile are sorted each within its
+    // own range by column (32-bit keys, a pass per 8 column bits); many
+    // small windows (other block shapes) are sorted as one range by
+    // (window, column) (64-bit keys), where per-window tiles would be mostly
+    // empty.
+    const bool segmented = nnz_big / n_big >= seg_radix_tile_size();
     bool in_1 = false;
-    const int key_bits = colbits + bits_for(n_big - 1);
-    st = radix_sort_pairs(ws.keys0, ws.vals0, ws.keys1, ws.vals1, nnz_big, key_bits, &in_1, ws.tmp,
-                          stream);
-    if (st != DTC_SUCCESS) return st;
-    st = scan_key_changes(in_1 ? ws.keys1 : ws.keys0, ws.gscan, nnz_big, ws.tmp, stream);
-    if (st != DTC_SUCCESS) return st;
-    const BigView bv = big_view(in_1 ? ws.keys1 : ws.keys0, in_1 ? ws.vals1 : ws.vals0, ws.gscan,
-                                ws.csb, wmap, colbits);
-    k_big_counts<<<1, 1024, 0, stream>>>(bv, n_big, blk_w, block_partition, ws.wU, ws.cbb);
-    DTC_LAUNCH_CHECK("big_counts");
+    BigView bv;
+    if (segmented) {
+      k_big_keys<uint32_t><<<grid_for(nnz_big, kBigChunk), 256, 0, gs>>>(
+          node_pointer, edge_list, num_nodes, blk_h, colbits, n_big, nnz_big, wmap, ws.csb,
+          edge_to_row, ws.k32_0, ws.vals0, ws.wpos);
+      DTC_LAUNCH_CHECK("big_keys");
+      const int64_t max_tiles = ceil_div(nnz_big, seg_radix_tile_size()) + n_big;
+      st = seg_radix_sort_pairs_u32(ws.k32_0, ws.vals0, ws.k32_1, ws.vals1, colbits, ws.tw, ws.tb,
+                                    ws.csb, ws.tb + n_big, max_tiles, &in_1, ws.tmp, gs);
+      if (st != DTC_SUCCESS) return st;
+      const uint32_t *sorted = in_1 ? ws.k32_1 : ws.k32_0;
+      st = scan_key_changes_u32(sorted, ws.gscan, nnz_big, ws.tmp, gs);
+      if (st != DTC_SUCCESS) return st;
+      bv = big_view(sorted, nullptr, colbits, in_1 ? ws.vals1 : ws.vals0, ws.gscan, ws.csb, wmap,
+                    ws.wpos);
+    } else {
+      k_big_keys<uint64_t><<<grid_for(nnz_big, kBigChunk), 256, 0, gs>>>(
+          node_pointer, edge_list, num_nodes, blk_h, colbits, n_big, nnz_big, wmap, ws.csb,
+          edge_to_row, ws.keys0, ws.vals0, ws.wpos);
+      DTC_LAUNCH_CHECK("big_keys");
+      const int key_bits = colbits + bits_for(n_big - 1);
+      st = radix_sort_pairs(ws.keys0, ws.vals0, ws.keys1, ws.vals1, nnz_big, key_bits, &in_1,
+                            ws.tmp, gs);
+      if (st != DTC_SUCCESS) return st;
+      const uint64_t *sorted = in_1 ? ws.keys1 : ws.keys0;
+      st = scan_key_changes(sorted, ws.gscan, nnz_big, ws.tmp, gs);
+      if (st != DTC_SUCCESS) return st;
+      bv = big_view(nullptr, sorted, colbits, in_1 ? ws.vals1 : ws.vals0, ws.gscan, ws.csb, wmap,
+                    ws.wpos);
+    }
+    if (n_big <= kOneGroupWindows) {
+      k_big_counts<<<1, 1024, 0, gs>>>(bv, n_big, blk_w, block_partition, ws.wU, ws.cbb);
+      DTC_LAUNCH_CHECK("big_counts");
+    } else {
+      k_big_counts_multi<<<grid_for(n_big + 1, 256), 256, 0, gs>>>(bv, n_big, blk_w,
+                                                                       block_partition, ws.wU,
+                                                                       ws.cbb);
+      DTC_LAUNCH_CHECK("big_counts");
+      st = scan_i32(ws.cbb, ws.cbb, n_big + 1, false, ws.tmp, gs);
+      if (st != DTC_SUCCESS) return st;
+    }
     const unsigned grid = grid_for(nnz_big, 256);
-    k_big_stage<<<grid, 256, 0, stream>>>(node_pointer, num_nodes, blk_h, blk_w, nnz_big, bv,
+    k_big_stage<<<grid, 256, 0, gs>>>(node_pointer, num_nodes, blk_h, blk_w, nnz_big, bv,
                                           ws.cbb, edge_to_column, win_stage(ws), ws.bstart);
     DTC_LAUNCH_CHECK("big_stage");
-    k_big_tile_ids<<<grid, 256, 0, stream>>>(node_pointer, edge_to_row, num_nodes, blk_h, blk_w,
+    k_big_tile_ids<<<grid, 256, 0, gs>>>(node_pointer, edge_to_row, num_nodes, blk_h, blk_w,
                                              nnz_big, bv, ws.cbb, ws.bstart, values,
                                              tcblocktile_id, tcblock_values, tcblock_edge);
     DTC_LAUNCH_CHECK("big_tile_ids");
   }
+  if (side) DTC_HIP_CHECK(hipEventRecord(side->join, gs));
 
-  // 4. Window offsets (row_window_offset[0] was written by k_classify).
+  if (side) {
+    DTC_HIP_CHECK(hipStreamWaitEvent(stream, side->join, 0));
+    side_lock.unlock();
+  }
+
+  // 5. Window offsets (row_window_offset[0] was written by k_classify).
   st = scan_i32(block_partition, row_window_offset + 1, W, true, ws.tmp, stream);
   if (st != DTC_SUCCESS) return st;
 
-  // 5. Block count (second host synchronisation).
+  // 6. Block count (second host synchronisation).
   int32_t T = 0;
   DTC_HIP_CHECK(hipMemcpyAsync(&host, ws.hdr, 64, hipMemcpyDeviceToHost, stream));
   DTC_HIP_CHECK(hipMemcpyAsync(&T, row_window_offset + W, sizeof(int32_t), hipMemcpyDeviceToHost,

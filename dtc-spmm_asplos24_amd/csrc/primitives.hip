@@ -28,8 +28,9 @@ struct LoadI32 {
   __device__ __forceinline__ int operator()(int64_t i) const { return p[i]; }
 };
 
+template <class Key>
 struct LoadKeyChange {
-  const uint64_t *k;
+  const Key *k;
   __device__ __forceinline__ int operator()(int64_t i) const {
     return (i == 0 || k[i] != k[i - 1]) ? 1 : 0;
   }
@@ -132,7 +133,12 @@ dtc_status_t scan_i32(const int32_t *in, int32_t *out, int64_t n, bool inclusive
 
 dtc_status_t scan_key_changes(const uint64_t *keys, int32_t *out, int64_t n, void *ws,
                               hipStream_t stream) {
-  return scan_impl(LoadKeyChange{keys}, out, n, true, ws, stream);
+  return scan_impl(LoadKeyChange<uint64_t>{keys}, out, n, true, ws, stream);
+}
+
+dtc_status_t scan_key_changes_u32(const uint32_t *keys, int32_t *out, int64_t n, void *ws,
+                                  hipStream_t stream) {
+  return scan_impl(LoadKeyChange<uint32_t>{keys}, out, n, true, ws, stream);
 }
 
 // -------------------------------------------------------------- radix sort
@@ -142,22 +148,67 @@ constexpr int kSortItems = 16;
 constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr int kRadix = 256;
 
-// hist[d * ntiles + tile] = number of keys in the tile whose digit is d.
-__global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint64_t *keys,
-                                                                   int64_t n, int shift,
-                                                                   int32_t *hist,
-                                                                   int ntiles) {
+// Tile maps: tile t covers [start, start + len) of the key array, and the
+// count of digit d in it lives at hist[hbase + d * hstride].
+//   PlainTiles: tiles of kSortTile keys over the whole array, hist laid out
+//     [digit][tile], so one exclusive scan of hist gives every (digit, tile)
+//     its first output position;
+//   SegTiles: the array is a sequence of segments (segment w' = [csb[w'],
+//     csb[w'+1])), each cut into its own tiles (tile t belongs to segment
+//     tw[t], segment w' owns tiles [tb[w'], tb[w'+1])), hist laid out
+//     [segment][digit][tile of the segment]: the same scan then gives
+//     csb[w'] + (entries of w' with a smaller digit) + (entries of digit d in
+//     earlier tiles of w'), i.e. each pass sorts every segment on its own.
+struct PlainTiles {
+  int64_t n;
+  int ntiles;
+  __device__ __forceinline__ bool valid(int) const { return true; }
+  __device__ __forceinline__ void get(int t, int64_t &start, int &len, int64_t &hbase,
+                                      int &hstride) const {
+    start = (int64_t)t * kSortTile;
+    len = (int)min<int64_t>(kSortTile, n - start);
+    hbase = t;
+    hstride = ntiles;
+  }
+};
+
+struct SegTiles {
+  const int32_t *tw, *tb, *csb;
+  const int32_t *ntot;  // number of tiles (the grid may be larger)
+  __device__ __forceinline__ bool valid(int t) const { return t < *ntot; }
+  __device__ __forceinline__ void get(int t, int64_t &start, int &len, int64_t &hbase,
+                                      int &hstride) const {
+    const int w = tw[t];
+    const int t0 = tb[w];
+    const int tl = t - t0;
+    start = (int64_t)csb[w] + (int64_t)tl * kSortTile;
+    len = (int)min<int64_t>(kSortTile, (int64_t)csb[w + 1] - start);
+    hbase = (int64_t)t0 * kRadix + tl;
+    hstride = tb[w + 1] - t0;
+  }
+};
+
+// Digit counts of one tile.
+template <class Key, class Tiles>
+__global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const Key *keys, int shift,
+                                                                   int32_t *hist, Tiles tiles) {
   __shared__ int cnt[kRadix];
+  if (!tiles.valid(blockIdx.x)) {  // past the last tile: the scanned tail reads zeros
+    hist[(int64_t)blockIdx.x * kRadix + threadIdx.x] = 0;
+    return;
+  }
   cnt[threadIdx.x] = 0;
   __syncthreads();
-  const int64_t base = (int64_t)blockIdx.x * kSortTile;
+  int64_t start, hbase;
+  int len, hstride;
+  tiles.get(blockIdx.x, start, len, hbase, hstride);
 #pragma unroll 4
   for (int k = 0; k < kSortItems; ++k) {
-    const int64_t i = base + k * kSortThreads + threadIdx.x;
-    if (i < n) atomicAdd(&cnt[(int)((keys[i] >> shift) & (kRadix - 1))], 1);
+    const int i = k * kSortThreads + threadIdx.x;
+    if (i < len) atomicAdd(&cnt[(int)((keys[start + i] >> shift) & (kRadix - 1))], 1);
   }
   __syncthreads();
-  hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
+  hist[hbase + (int64_t)threadIdx.x * hstride] = cnt[threadIdx.x];
 }
 
 // Stable scatter: element i of the tile goes to offsets[d][tile] + (number of
@@ -178,42 +229,43 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint64_t
 constexpr int kSortWaves = kSortThreads / 64;
 constexpr int kWaveItems = kSortTile / kSortWaves;  // 1,024 elements per wave
 
+template <class Key, class Tiles>
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
-    const uint64_t *kin, const int32_t *vin, uint64_t *kout, int32_t *vout, int64_t n,
-    int shift, const int32_t *offsets, int ntiles) {
+    const Key *kin, const int32_t *vin, Key *kout, int32_t *vout, int shift,
+    const int32_t *offsets, Tiles tiles) {
   __shared__ int s_off[kRadix];             // global position of the tile's first element of digit d
   __shared__ int s_loc[kRadix];             // local position of the same element
   __shared__ int s_wcnt[kSortWaves][kRadix];  // per-wave digit counts, then per-wave prefixes
   __shared__ int s_scan[kSortWaves];
-  __shared__ uint64_t s_keys[kSortTile];
+  __shared__ Key s_keys[kSortTile];
   __shared__ int32_t s_vals[kSortTile];
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
   const int lane = tid & 63;
   const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-  const int64_t base = (int64_t)blockIdx.x * kSortTile;
-  const int64_t wbase = base + (int64_t)wave * kWaveItems;
+  if (!tiles.valid(blockIdx.x)) return;  // workgroup-uniform
+  int64_t start, hbase;
+  int len, hstride;
+  tiles.get(blockIdx.x, start, len, hbase, hstride);
+  const int wbase = wave * kWaveItems;  // tile-relative
   // All of the wave's keys and values are loaded up front (one memory
   // latency for the whole tile).
-  uint64_t kreg[kSortItems];
+  Key kreg[kSortItems];
   int32_t vreg[kSortItems];
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
-    const int64_t i = wbase + r * 64 + lane;
-    kreg[r] = i < n ? kin[i] : 0ull;
-    vreg[r] = i < n ? vin[i] : 0;
+    const int i = wbase + r * 64 + lane;
+    kreg[r] = i < len ? kin[start + i] : (Key)0;
+    vreg[r] = i < len ? vin[start + i] : 0;
   }
-  {
-    const int64_t idx = (int64_t)tid * ntiles + blockIdx.x;
-    s_off[tid] = offsets[idx];
-  }
+  s_off[tid] = offsets[hbase + (int64_t)tid * hstride];
 #pragma unroll
   for (int k = 0; k < kRadix / 64; ++k) s_wcnt[wave][k * 64 + lane] = 0;
   wave_lds_sync();
   int lrank[kSortItems];  // rank among the wave's earlier elements of the same digit
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
-    const bool valid = wbase + r * 64 + lane < n;
+    const bool valid = wbase + r * 64 + lane < len;
     const int d = (int)((kreg[r] >> shift) & (kRadix - 1));
     uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -248,7 +300,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
-    if (wbase + r * 64 + lane < n) {
+    if (wbase + r * 64 + lane < len) {
       const int d = (int)((kreg[r] >> shift) & (kRadix - 1));
       const int lp = s_loc[d] + s_wcnt[wave][d] + lrank[r];
       s_keys[lp] = kreg[r];
@@ -256,9 +308,8 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     }
   }
   __syncthreads();
-  const int tile_n = (int)(n - base < kSortTile ? n - base : kSortTile);
-  for (int j = tid; j < tile_n; j += kSortThreads) {
-    const uint64_t key = s_keys[j];
+  for (int j = tid; j < len; j += kSortThreads) {
+    const Key key = s_keys[j];
     const int d = (int)((key >> shift) & (kRadix - 1));
     const int64_t pos = (int64_t)s_off[d] + (j - s_loc[d]);
     kout[pos] = key;
@@ -285,14 +336,55 @@ dtc_status_t radix_sort_pairs(uint64_t *keys0, int32_t *vals0, uint64_t *keys1,
   uint64_t *kin = keys0, *kout = keys1;
   int32_t *vin = vals0, *vout = vals1;
   bool in_1 = false;
+  const PlainTiles tiles{n, (int)ntiles};
   for (int shift = 0; shift < key_bits; shift += 8) {
-    radix_hist_kernel<<<(unsigned)ntiles, kSortThreads, 0, stream>>>(kin, n, shift, hist,
-                                                                      (int)ntiles);
+    radix_hist_kernel<uint64_t, PlainTiles>
+        <<<(unsigned)ntiles, kSortThreads, 0, stream>>>(kin, shift, hist, tiles);
     DTC_LAUNCH_CHECK("radix_hist");
     dtc_status_t st = scan_i32(hist, hist, ntiles * kRadix, false, scan_ws, stream);
     if (st != DTC_SUCCESS) return st;
-    radix_scatter_kernel<<<(unsigned)ntiles, kSortThreads, 0, stream>>>(
-        kin, vin, kout, vout, n, shift, hist, (int)ntiles);
+    radix_scatter_kernel<uint64_t, PlainTiles>
+        <<<(unsigned)ntiles, kSortThreads, 0, stream>>>(kin, vin, kout, vout, shift, hist, tiles);
+    DTC_LAUNCH_CHECK("radix_scatter");
+    std::swap(kin, kout);
+    std::swap(vin, vout);
+    in_1 = !in_1;
+  }
+  *result_in_1 = in_1;
+  return DTC_SUCCESS;
+}
+
+size_t seg_radix_sort_workspace_bytes(int64_t n, int64_t segments) {
+  const int64_t ntiles = std::max<int64_t>(1, ceil_div(n, kSortTile) + segments);
+  return align_up(sizeof(int32_t) * (size_t)(ntiles * kRadix), 256) +
+         scan_workspace_bytes(ntiles * kRadix);
+}
+
+int64_t seg_radix_tile_size() { return kSortTile; }
+
+dtc_status_t seg_radix_sort_pairs_u32(uint32_t *keys0, int32_t *vals0, uint32_t *keys1,
+                                      int32_t *vals1, int key_bits, const int32_t *tw,
+                                      const int32_t *tb, const int32_t *csb,
+                                      const int32_t *ntiles_dev, int64_t ntiles,
+                                      bool *result_in_1, void *ws, hipStream_t stream) {
+  *result_in_1 = false;
+  if (ntiles <= 0 || key_bits <= 0) return DTC_SUCCESS;
+  if (ntiles * kRadix > INT32_MAX) return fail(DTC_ERROR_UNSUPPORTED, "sort too large");
+  Carver c(ws);
+  int32_t *hist = c.take<int32_t>((size_t)(ntiles * kRadix));
+  void *scan_ws = c.take<char>(scan_workspace_bytes(ntiles * kRadix));
+  uint32_t *kin = keys0, *kout = keys1;
+  int32_t *vin = vals0, *vout = vals1;
+  bool in_1 = false;
+  const SegTiles tiles{tw, tb, csb, ntiles_dev};
+  for (int shift = 0; shift < key_bits; shift += 8) {
+    radix_hist_kernel<uint32_t, SegTiles>
+        <<<(unsigned)ntiles, kSortThreads, 0, stream>>>(kin, shift, hist, tiles);
+    DTC_LAUNCH_CHECK("radix_hist");
+    dtc_status_t st = scan_i32(hist, hist, ntiles * kRadix, false, scan_ws, stream);
+    if (st != DTC_SUCCESS) return st;
+    radix_scatter_kernel<uint32_t, SegTiles>
+        <<<(unsigned)ntiles, kSortThreads, 0, stream>>>(kin, vin, kout, vout, shift, hist, tiles);
     DTC_LAUNCH_CHECK("radix_scatter");
     std::swap(kin, kout);
     std::swap(vin, vout);
