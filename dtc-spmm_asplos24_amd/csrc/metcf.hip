@@ -33,6 +33,7 @@
 // Both paths write every output of a window and nothing of another window,
 // so the classification changes no result.
 #include <algorithm>
+#include <cstddef>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -61,15 +62,17 @@ constexpr int kPathAuto = 0;
 constexpr int kPathGlobal = 1;
 constexpr int kPathClass0 = 2;
 
+constexpr int kMaxSlots = 16;  // the column maximum is spread over this many words
 struct MetcfHeader {
   int32_t flags;    // CSR check bits, see kBad*
-  int32_t max_col;  // largest column index
   int32_t nnz_big;  // entries of the global-path windows
-  int32_t pad0;
   int32_t cls_count[kClasses + 1];  // windows per LDS class; [kClasses]: global-path windows
-  int32_t pad[53];
+  int32_t max_col[kMaxSlots];       // largest column index: the maximum of these
+  int32_t pad[64 - 2 - (kClasses + 1) - kMaxSlots];
 };
+constexpr int kHeaderRead = 128;  // bytes of the header read by the host
 static_assert(sizeof(MetcfHeader) == 256, "header size");
+static_assert(offsetof(MetcfHeader, pad) <= kHeaderRead, "header read size");
 
 constexpr int kBadFirst = 1;     // rowptr[0] != 0
 constexpr int kBadLast = 2;      // rowptr[M] != nnz
@@ -151,9 +154,9 @@ int path_mode(int32_t blk_h, int32_t blk_w) {
 __global__ void k_header_init(MetcfHeader *h) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     h->flags = 0;
-    h->max_col = 0;
     h->nnz_big = 0;
     for (int c = 0; c <= kClasses; ++c) h->cls_count[c] = 0;
+    for (int c = 0; c < kMaxSlots; ++c) h->max_col[c] = 0;
   }
 }
 
@@ -210,7 +213,8 @@ __global__ __launch_bounds__(256) void k_validate(const int32_t *rowptr, const i
   if (threadIdx.x == 0) {
     const int bf = s_f[0] | s_f[1] | s_f[2] | s_f[3];
     if (bf) atomicOr(&h->flags, bf);
-    atomicMax(&h->max_col, max(max(s_m[0], s_m[1]), max(s_m[2], s_m[3])));
+    // spread over kMaxSlots words: atomics on one word serialise in L2
+    atomicMax(&h->max_col[blockIdx.x % kMaxSlots], max(max(s_m[0], s_m[1]), max(s_m[2], s_m[3])));
   }
 }
 
@@ -330,11 +334,18 @@ __device__ __forceinline__ int wg1024_exclusive_scan(Load load, Store store, int
 // Also cuts each window into sort tiles of `tile` entries: tb[k] = tiles of
 // the windows listed before k, tb[n_big] = all of them, tw[t] = window of
 // tile t.
+//
+// Runs before the host knows the window count: reads it from the header and
+// leaves the work to the multi-workgroup kernels when it exceeds
+// kOneGroupWindows.
+constexpr int kOneGroupWindows = 4096;  // up to this many: one-workgroup kernels
 __global__ __launch_bounds__(1024) void k_big_offsets(const int32_t *rowptr, int32_t M,
                                                       int32_t blk_h, const int32_t *wmap,
-                                                      int32_t n_big, int32_t tile, int32_t *csb,
-                                                      int32_t *tb, int32_t *tw) {
+                                                      const MetcfHeader *h, int32_t tile,
+                                                      int32_t *csb, int32_t *tb, int32_t *tw) {
   __shared__ int s_tmp[16];
+  const int n_big = h->cls_count[kClasses];
+  if (n_big == 0 || n_big > kOneGroupWindows) return;  // workgroup-uniform
   auto entries = [&](int i) {
     const int w = wmap[i];
     return window_end(rowptr, w, blk_h, M) - window_begin(rowptr, w, blk_h, M);
@@ -354,7 +365,6 @@ __global__ __launch_bounds__(1024) void k_big_offsets(const int32_t *rowptr, int
 // Many global-path windows: the same offsets with one thread per window
 // and device-wide scans (csb and tb hold the counts, with a trailing 0, until
 // scanned), then the tile map.
-constexpr int kOneGroupWindows = 4096;  // up to this many: one-workgroup kernels
 __global__ __launch_bounds__(256) void k_big_sizes(const int32_t *rowptr, int32_t M,
                                                    int32_t blk_h, const int32_t *wmap,
                                                    int32_t n_big, int32_t tile, int32_t *csb,
@@ -899,6 +909,7 @@ __global__ __launch_bounds__(256) void k_blocks_out(const int32_t *__restrict__ 
   if (in_lds)
     for (int k = tid; k <= wb - wa + 1; k += blockDim.x) s_rwo[k] = rwo[wa + k];
   __syncthreads();
+#pragma unroll 2
   for (int t = tid; t < nblk; t += blockDim.x) {
     const int b = first + t;
     int w;
@@ -924,6 +935,7 @@ __global__ __launch_bounds__(256) void k_blocks_out(const int32_t *__restrict__ 
   __syncthreads();
   const int n_atox = nblk * blk_w;
   const int64_t a0 = (int64_t)first * blk_w;
+#pragma unroll 4
   for (int e = tid; e < n_atox; e += blockDim.x) {
     const int t = e / blk_w, c = e - t * blk_w;
     atox[a0 + e] = s_kb[t] + c < s_U[t] ? st.dcol[s_base[t] + c] : 0;
@@ -1185,6 +1197,10 @@ struct SideStream {
   hipStream_t s = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
   std::mutex mu;
+  // pinned host words for the build's device-to-host reads (header, block
+  // count), guarded by `read_mu`
+  char *pinned = nullptr;
+  std::mutex read_mu;
 };
 
 dtc_status_t side_stream(SideStream **out) {
@@ -1199,6 +1215,7 @@ dtc_status_t side_stream(SideStream **out) {
     DTC_HIP_CHECK(hipStreamCreateWithFlags(&x->s, hipStreamNonBlocking));
     DTC_HIP_CHECK(hipEventCreateWithFlags(&x->fork, hipEventDisableTiming));
     DTC_HIP_CHECK(hipEventCreateWithFlags(&x->join, hipEventDisableTiming));
+    DTC_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&x->pinned), 256, hipHostMallocDefault));
     per_dev[dev] = x;
   }
   *out = per_dev[dev];
@@ -1276,10 +1293,25 @@ dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *no
                                                       mode, block_partition, ws.lists,
                                                       row_window_offset, ws.hdr);
     DTC_LAUNCH_CHECK("classify");
+    // offsets and sort tiles of up to kOneGroupWindows global-path windows
+    k_big_offsets<<<1, 1024, 0, stream>>>(node_pointer, num_nodes, blk_h,
+                                          ws.lists + (int64_t)kClasses * W, ws.hdr,
+                                          (int32_t)seg_radix_tile_size(), ws.csb, ws.tb, ws.tw);
+    DTC_LAUNCH_CHECK("big_offsets");
   }
+  SideStream *dev_state = nullptr;
+  st = side_stream(&dev_state);
+  if (st != DTC_SUCCESS) return st;
   MetcfHeader host{};
-  DTC_HIP_CHECK(hipMemcpyAsync(&host, ws.hdr, 64, hipMemcpyDeviceToHost, stream));
-  DTC_HIP_CHECK(hipStreamSynchronize(stream));
+  {
+    std::lock_guard<std::mutex> g(dev_state->read_mu);
+    DTC_HIP_CHECK(hipMemcpyAsync(dev_state->pinned, ws.hdr, kHeaderRead, hipMemcpyDeviceToHost,
+                                 stream));
+    DTC_HIP_CHECK(hipStreamSynchronize(stream));
+    std::memcpy(&host, dev_state->pinned, kHeaderRead);
+  }
+  int max_col = 0;
+  for (int c = 0; c < kMaxSlots; ++c) max_col = std::max(max_col, host.max_col[c]);
   if (host.flags) {
     std::string why;
     if (host.flags & kBadFirst) why += " nodePointer[0] != 0;";
@@ -1297,28 +1329,61 @@ dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *no
     *block_count = 0;
     return DTC_SUCCESS;
   }
-  const int colbits = std::max(1, bits_for(host.max_col));
+  const int colbits = std::max(1, bits_for(max_col));
   const int n_big = host.cls_count[kClasses];
   const int nnz_big = host.nnz_big;
   const int32_t *wmap = ws.lists + (int64_t)kClasses * W;
 
   // 2. Fork: the global path goes to the side stream when the LDS path also
-  //    has work.  The LDS launches are issued first (few, long) so that they
-  //    start while the host issues the global path's many short ones.
+  //    has work.  Issue order: the global path's first kernel, the few long
+  //    LDS launches, then the global path's many short ones, so that both
+  //    streams start at once (the global path is the longer of the two).
   int n_lds = 0;
   for (int c = 0; c < kClasses; ++c) n_lds += host.cls_count[c];
   SideStream *side = nullptr;
   std::unique_lock<std::mutex> side_lock;
   hipStream_t gs = stream;
   if (nnz_big > 0 && n_lds > 0) {
-    st = side_stream(&side);
-    if (st != DTC_SUCCESS) return st;
+    side = dev_state;
     side_lock = std::unique_lock<std::mutex>(side->mu);
     gs = side->s;
     DTC_HIP_CHECK(hipEventRecord(side->fork, stream));
     DTC_HIP_CHECK(hipStreamWaitEvent(gs, side->fork, 0));
   }
-  // 3. LDS path, largest class first: every output of the window sized by
+  // 3. Global path, first steps (issued before the LDS path so that the
+  //    side stream starts at once): window offsets and sort tiles of many
+  //    windows; keys.  Windows averaging at least one sort tile are sorted
+  //    each within its own range by column (32-bit keys, a pass per 8 column
+  //    bits); many small windows (other block shapes) are sorted as one
+  //    range by (window, column) (64-bit keys), where per-window tiles would
+  //    be mostly empty.
+  if (n_big > kOneGroupWindows) {
+    k_big_sizes<<<grid_for(n_big + 1, 256), 256, 0, gs>>>(
+        node_pointer, num_nodes, blk_h, wmap, n_big, (int32_t)seg_radix_tile_size(), ws.csb,
+        ws.tb);
+    DTC_LAUNCH_CHECK("big_sizes");
+    st = scan_i32(ws.csb, ws.csb, n_big + 1, false, ws.tmp, gs);
+    if (st != DTC_SUCCESS) return st;
+    st = scan_i32(ws.tb, ws.tb, n_big + 1, false, ws.tmp, gs);
+    if (st != DTC_SUCCESS) return st;
+    k_big_tilemap<<<grid_for(n_big, 256), 256, 0, gs>>>(ws.tb, n_big, ws.tw);
+    DTC_LAUNCH_CHECK("big_tilemap");
+  }
+
+  const bool segmented = n_big > 0 && nnz_big / n_big >= seg_radix_tile_size();
+  if (nnz_big > 0 && segmented) {
+    k_big_keys<uint32_t><<<grid_for(nnz_big, kBigChunk), 256, 0, gs>>>(
+        node_pointer, edge_list, num_nodes, blk_h, colbits, n_big, nnz_big, wmap, ws.csb,
+        edge_to_row, ws.k32_0, ws.vals0, ws.wpos);
+    DTC_LAUNCH_CHECK("big_keys");
+  } else if (nnz_big > 0) {
+    k_big_keys<uint64_t><<<grid_for(nnz_big, kBigChunk), 256, 0, gs>>>(
+        node_pointer, edge_list, num_nodes, blk_h, colbits, n_big, nnz_big, wmap, ws.csb,
+        edge_to_row, ws.keys0, ws.vals0, ws.wpos);
+    DTC_LAUNCH_CHECK("big_keys");
+  }
+
+  // 4. LDS path, largest class first: every output of the window sized by
   //    its entries, and its block data for phase 2.
   for (int c = kClasses - 1; c >= 0 && st == DTC_SUCCESS; --c) {
     const int n = host.cls_count[c];
@@ -1341,40 +1406,12 @@ dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *no
     }
   }
   if (st != DTC_SUCCESS) return st;
-  // 4. Global path: window offsets and sort tiles; keys, sort, unique-column
-  //    numbering; blockPartition, edgeToColumn, block data, tile ids; then
-  //    join.
-  if (n_big > 0 && n_big <= kOneGroupWindows) {
-    k_big_offsets<<<1, 1024, 0, gs>>>(node_pointer, num_nodes, blk_h, wmap, n_big,
-                                          (int32_t)seg_radix_tile_size(), ws.csb, ws.tb, ws.tw);
-    DTC_LAUNCH_CHECK("big_offsets");
-  } else if (n_big > 0) {
-    k_big_sizes<<<grid_for(n_big + 1, 256), 256, 0, gs>>>(
-        node_pointer, num_nodes, blk_h, wmap, n_big, (int32_t)seg_radix_tile_size(), ws.csb,
-        ws.tb);
-    DTC_LAUNCH_CHECK("big_sizes");
-    st = scan_i32(ws.csb, ws.csb, n_big + 1, false, ws.tmp, gs);
-    if (st != DTC_SUCCESS) return st;
-    st = scan_i32(ws.tb, ws.tb, n_big + 1, false, ws.tmp, gs);
-    if (st != DTC_SUCCESS) return st;
-    k_big_tilemap<<<grid_for(n_big, 256), 256, 0, gs>>>(ws.tb, n_big, ws.tw);
-    DTC_LAUNCH_CHECK("big_tilemap");
-  }
-
+  // 5. Global path, continued: sort, unique-column numbering;
+  //    blockPartition, edgeToColumn, block data, tile ids; then join.
   if (nnz_big > 0) {
-    // Windows averaging at least one sort tile are sorted each within its
-    // own range by column (32-bit keys, a pass per 8 column bits); many
-    // small windows (other block shapes) are sorted as one range by
-    // (window, column) (64-bit keys), where per-window tiles would be mostly
-    // empty.
-    const bool segmented = nnz_big / n_big >= seg_radix_tile_size();
     bool in_1 = false;
     BigView bv;
     if (segmented) {
-      k_big_keys<uint32_t><<<grid_for(nnz_big, kBigChunk), 256, 0, gs>>>(
-          node_pointer, edge_list, num_nodes, blk_h, colbits, n_big, nnz_big, wmap, ws.csb,
-          edge_to_row, ws.k32_0, ws.vals0, ws.wpos);
-      DTC_LAUNCH_CHECK("big_keys");
       const int64_t max_tiles = ceil_div(nnz_big, seg_radix_tile_size()) + n_big;
       st = seg_radix_sort_pairs_u32(ws.k32_0, ws.vals0, ws.k32_1, ws.vals1, colbits, ws.tw, ws.tb,
                                     ws.csb, ws.tb + n_big, max_tiles, &in_1, ws.tmp, gs);
@@ -1385,10 +1422,6 @@ dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *no
       bv = big_view(sorted, nullptr, colbits, in_1 ? ws.vals1 : ws.vals0, ws.gscan, ws.csb, wmap,
                     ws.wpos);
     } else {
-      k_big_keys<uint64_t><<<grid_for(nnz_big, kBigChunk), 256, 0, gs>>>(
-          node_pointer, edge_list, num_nodes, blk_h, colbits, n_big, nnz_big, wmap, ws.csb,
-          edge_to_row, ws.keys0, ws.vals0, ws.wpos);
-      DTC_LAUNCH_CHECK("big_keys");
       const int key_bits = colbits + bits_for(n_big - 1);
       st = radix_sort_pairs(ws.keys0, ws.vals0, ws.keys1, ws.vals1, nnz_big, key_bits, &in_1,
                             ws.tmp, gs);
@@ -1426,16 +1459,22 @@ dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *no
     side_lock.unlock();
   }
 
-  // 5. Window offsets (row_window_offset[0] was written by k_classify).
+  // 6. Window offsets (row_window_offset[0] was written by k_classify).
   st = scan_i32(block_partition, row_window_offset + 1, W, true, ws.tmp, stream);
   if (st != DTC_SUCCESS) return st;
 
-  // 6. Block count (second host synchronisation).
+  // 7. Block count (second host synchronisation).
   int32_t T = 0;
-  DTC_HIP_CHECK(hipMemcpyAsync(&host, ws.hdr, 64, hipMemcpyDeviceToHost, stream));
-  DTC_HIP_CHECK(hipMemcpyAsync(&T, row_window_offset + W, sizeof(int32_t), hipMemcpyDeviceToHost,
-                               stream));
-  DTC_HIP_CHECK(hipStreamSynchronize(stream));
+  {
+    std::lock_guard<std::mutex> g(dev_state->read_mu);
+    DTC_HIP_CHECK(hipMemcpyAsync(dev_state->pinned, ws.hdr, kHeaderRead, hipMemcpyDeviceToHost,
+                                 stream));
+    DTC_HIP_CHECK(hipMemcpyAsync(dev_state->pinned + kHeaderRead, row_window_offset + W,
+                                 sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    DTC_HIP_CHECK(hipStreamSynchronize(stream));
+    std::memcpy(&host, dev_state->pinned, kHeaderRead);
+    std::memcpy(&T, dev_state->pinned + kHeaderRead, sizeof(int32_t));
+  }
   if (host.flags & kBadClass)
     return fail(DTC_ERROR_INCONSISTENT, "internal error: a window exceeded its class capacity");
   *block_count = T;
