@@ -24,7 +24,8 @@ OBJ = os.path.join(HERE, "_build")
 INCLUDE = os.path.join(REPO, "include")
 HIP_SOURCES = ["primitives.hip", "metcf.hip", "spmm.hip", "backward.hip", "selector.hip",
                "reorder.hip", "gcn.hip"]
-HEADERS = [os.path.join(CSRC, "dtc_internal.h"), os.path.join(INCLUDE, "dtc.h")]
+HEADERS = [os.path.join(CSRC, "dtc_internal.h"), os.path.join(CSRC, "seg_sort.h"),
+           os.path.join(INCLUDE, "dtc.h")]
 LIB = os.path.join(HERE, "libdtc.so")
 # measurement aid beside the library (not part of include/dtc.h)
 PROBE_SRC = "probe_gather.hip"

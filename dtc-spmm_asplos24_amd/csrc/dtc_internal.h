@@ -217,21 +217,6 @@ dtc_status_t scan_key_changes(const uint64_t *keys, int32_t *out, int64_t n, voi
 // Result lands in (keys0, vals0) if `*result_in_1` is false, else in
 // (keys1, vals1).
 size_t radix_sort_workspace_bytes(int64_t n);
-
-// Segmented variant over 32-bit keys: the array is segments [csb[w'],
-// csb[w'+1]), each cut into its own tiles of seg_radix_tile_size() keys
-// (tile t of segment tw[t]; segment w' owns tiles [tb[w'], tb[w'+1]),
-// *ntiles_dev in all, at most ntiles); every segment is sorted stably by key
-// bits [0, key_bits) on its own, in place of the whole array.
-size_t seg_radix_sort_workspace_bytes(int64_t n, int64_t segments);
-int64_t seg_radix_tile_size();
-dtc_status_t seg_radix_sort_pairs_u32(uint32_t *keys0, int32_t *vals0, uint32_t *keys1,
-                                      int32_t *vals1, int key_bits, const int32_t *tw,
-                                      const int32_t *tb, const int32_t *csb,
-                                      const int32_t *ntiles_dev, int64_t ntiles,
-                                      bool *result_in_1, void *ws, hipStream_t stream);
-dtc_status_t scan_key_changes_u32(const uint32_t *keys, int32_t *out, int64_t n, void *ws,
-                                  hipStream_t stream);
 dtc_status_t radix_sort_pairs(uint64_t *keys0, int32_t *vals0, uint64_t *keys1,
                               int32_t *vals1, int64_t n, int key_bits, bool *result_in_1,
                               void *ws, hipStream_t stream);

@@ -39,7 +39,7 @@
 #include <mutex>
 #include <string>
 
-#include "dtc_internal.h"
+#include "seg_sort.h"
 
 namespace dtc {
 namespace {

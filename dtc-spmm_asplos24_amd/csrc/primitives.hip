@@ -3,7 +3,7 @@
 // (uint64 key, int32 value) pairs whose per-tile ranking uses wave ballots.
 #include <string>
 
-#include "dtc_internal.h"
+#include "seg_sort.h"
 
 namespace dtc {
 
