@@ -10,19 +10,22 @@
 // by entry count E:
 //
 //   LDS path (16-row windows of 8-column blocks, E <= 16,384): one workgroup
-//   per window (five shapes, 1 to 16 waves, up to 512 ... 16,384 entries)
+//   per window (five shapes, 1 to 16 waves, up to 512 ... 16,384 entries;
+//   windows of at most 64 entries take one wave each and bitonic sorts)
 //   sorts the window's (column, local index) pairs in LDS (LSD radix sort
-//   over the window's column span, 8-bit digits, ranks from LDS atomics in
-//   the first pass and from wave ballots in the stable later ones), numbers
-//   the distinct columns by a workgroup scan and writes every output sized
-//   by the CSR: edgeToRow, edgeToColumn, blockPartition, and the tile ids,
-//   values and edge indices in TC-block order, each entry at its block's
-//   start plus the number of the block's entries with a smaller CSR index
-//   (the reference's CSR-order placement inside a block, :291-300).
+//   over the window's column span, ranks from LDS atomics in the first pass
+//   and from wave ballots in the stable later ones), numbers the distinct
+//   columns by a workgroup scan and writes every output sized by the CSR:
+//   edgeToRow, edgeToColumn, blockPartition, and the tile ids, values and
+//   edge indices in TC-block order, each entry at its block's start plus
+//   the number of the block's entries with a smaller CSR index (the
+//   reference's CSR-order placement inside a block, :291-300).
 //
-//   Global path (larger windows, and every window of other block shapes):
-//   the windows are compacted, keyed (compacted window, column) and sorted
-//   by the device-wide radix sort; the inclusive scan G of "key differs from
+//   Global path (larger windows, and every window of other block shapes),
+//   on a side stream beside the LDS path: the windows are compacted and
+//   sorted by column, each within its own range (segmented radix sort of
+//   32-bit keys) when they average at least one sort tile, else together by
+//   (compacted window, column); the inclusive scan G of "key differs from
 //   its predecessor" gives every sorted element its unique-column number, so
 //   the rank of a column inside its window is G - G(window start), and
 //   edge-parallel kernels write the same outputs.
