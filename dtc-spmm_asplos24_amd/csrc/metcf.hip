@@ -511,10 +511,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_window_sort(
   constexpr int N1 = 1 << D1, N2 = 1 << D2;
   constexpr int kRaw = kWide ? CAP + CAP / 2 : CAP / 2;
   __shared__ uint32_t s_raw[kRaw];
-  __shared__ int s_cnt[win_cnt(WAVES)];
+  __shared__ __align__(16) int s_cnt[win_cnt(WAVES)];
   __shared__ uint16_t s_bs[CAP / kWinW + 1];
   __shared__ int s_rp[kWinH + 1];
   __shared__ int s_min[WAVES], s_max[WAVES], s_wtot[WAVES];
+  __shared__ int s_unsorted;
   __shared__ uint32_t s_last[WAVES];
   __shared__ int s_scan[16];
   uint32_t *s_k = s_raw;                                                 // kWide only
@@ -529,6 +530,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_window_sort(
     return;
   }
   if (tid <= kWinH) s_rp[tid] = rowptr[min((int64_t)w * kWinH + tid, (int64_t)M)] - s;
+  if (tid == 0) s_unsorted = 0;
   __syncthreads();
 
   // key[k]: column - window minimum.  ix[k]: local index (low 16 bits)
@@ -548,11 +550,24 @@ __global__ __launch_bounds__(64 * WAVES) void k_window_sort(
     cmin = q < E ? min(cmin, c) : cmin;
     cmax = q < E ? max(cmax, c) : cmax;
   }
+  // edgeToRow; and whether every row of the window is strictly increasing
+  // in column (sorted, no duplicates), which lets the TC-block positions
+  // below come from a row-major occupancy bitmap of each block
+  bool unsorted = false;
 #pragma unroll
   for (int k = 0; k < ITEMS; ++k) {
     const int q = vbase + k * 64 + lane;
-    if (q < E) e2r[s + q] = w * kWinH + row_of(s_rp, q);
+    const int up = __shfl_up((int)key[k], 1, 64);  // column of entry q - 1 (lanes 1..63)
+    if (q < E) {
+      const int r = row_of(s_rp, q);
+      e2r[s + q] = w * kWinH + r;
+      if (q > s_rp[r]) {  // not the first entry of its row
+        const int prev = lane > 0 ? up : col[s + q - 1];
+        unsorted |= (int)key[k] <= prev;
+      }
+    }
   }
+  if (__ballot(unsorted) != 0 && lane == 0) s_unsorted = 1;
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) {
     cmin = min(cmin, __shfl_xor(cmin, d, 64));
@@ -723,6 +738,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_window_sort(
     }
   }
   if (lane == 0) s_wtot[v] = run;
+  // the counters are free now: they become the blocks' occupancy bitmaps
+  for (int i = tid; i < win_cnt(WAVES); i += NT) s_cnt[i] = 0;
   __syncthreads();
   int pre = 0, U = 0;
 #pragma unroll
@@ -732,6 +749,23 @@ __global__ __launch_bounds__(64 * WAVES) void k_window_sort(
     U += t;
   }
   const int nb = (U + kWinW - 1) / kWinW;
+  // Bitmap path: block kb's 16 x 8 (row, slot) occupancy in words
+  // s_cnt[4 kb .. 4 kb + 3].  With strictly increasing rows, the CSR order of
+  // a block's entries is the row-major (row, slot) order, so an entry's
+  // place in its block is the number of set bits before its own.
+  const bool bitmap = s_unsorted == 0 && nb * 4 <= win_cnt(WAVES);
+  if (bitmap) {
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      const int q = vbase + k * 64 + lane;
+      if (vbase + k * 64 < E && q < E) {
+        const int rank = (int)(ix[k] >> 16) + pre - 1;
+        const int bit = row_of(s_rp, (int)(ix[k] & 0xffffu)) * kWinW + rank % kWinW;
+        atomicOr(reinterpret_cast<unsigned *>(&s_cnt[(rank / kWinW) * 4 + (bit >> 5)]),
+                 1u << (bit & 31));
+      }
+    }
+  }
 #pragma unroll
   for (int k = 0; k < ITEMS; ++k) {
     if ((newmask >> k) & 1u) {
@@ -758,11 +792,24 @@ __global__ __launch_bounds__(64 * WAVES) void k_window_sort(
       const int rank = (int)(ix[k] >> 16) + pre - 1;
       const uint32_t i = ix[k] & 0xffffu;
       const int kb = rank / kWinW, c = rank % kWinW;
-      const int lo = s_bs[kb], hi = s_bs[kb + 1];
+      const int lo = s_bs[kb];
+      const int rr = row_of(s_rp, (int)i);
       int pos = lo;
-      for (int j = lo; j < hi; ++j) pos += (uint32_t)s_x[j] < i ? 1 : 0;
+      if (bitmap) {
+        const int bit = rr * kWinW + c;
+        const int wi = bit >> 5;
+        const uint32_t below = (1u << (bit & 31)) - 1u;
+        const int4 m = *reinterpret_cast<const int4 *>(&s_cnt[kb * 4]);
+        pos += __popc(wi == 0 ? ((uint32_t)m.x & below) : (uint32_t)m.x);
+        pos += wi >= 1 ? __popc(wi == 1 ? ((uint32_t)m.y & below) : (uint32_t)m.y) : 0;
+        pos += wi >= 2 ? __popc(wi == 2 ? ((uint32_t)m.z & below) : (uint32_t)m.z) : 0;
+        pos += wi >= 3 ? __popc((uint32_t)m.w & below) : 0;
+      } else {
+        const int hi = s_bs[kb + 1];
+        for (int j = lo; j < hi; ++j) pos += (uint32_t)s_x[j] < i ? 1 : 0;
+      }
       if (pos < E) {
-        tile[s + pos] = (uint8_t)(row_of(s_rp, (int)i) * kWinW + c);
+        tile[s + pos] = (uint8_t)(rr * kWinW + c);
         if (tcvalues) tcvalues[s + pos] = values ? values[s + i] : 1.0f;
         if (tcedge) tcedge[s + pos] = s + (int)i;
       }
