@@ -507,6 +507,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_window_sort(
   constexpr int NT = 64 * WAVES;
   constexpr int CAP = NT * ITEMS;
   constexpr bool kWide = win_wide(WAVES, ITEMS);
+  // the bitmap path for TC-block positions, in the shapes of the largest
+  // windows (where blocks fill up); the smaller shapes count per entry
+  constexpr bool kBitmap = WAVES >= 8;
   constexpr int D1 = win_d1(WAVES), D2 = win_d2(WAVES);
   constexpr int N1 = 1 << D1, N2 = 1 << D2;
   constexpr int kRaw = kWide ? CAP + CAP / 2 : CAP / 2;
@@ -515,7 +518,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_window_sort(
   __shared__ uint16_t s_bs[CAP / kWinW + 1];
   __shared__ int s_rp[kWinH + 1];
   __shared__ int s_min[WAVES], s_max[WAVES], s_wtot[WAVES];
-  __shared__ int s_unsorted;
+  __shared__ int s_unsorted;  // bitmap path: some row not strictly increasing
   __shared__ uint32_t s_last[WAVES];
   __shared__ int s_scan[16];
   uint32_t *s_k = s_raw;                                                 // kWide only
@@ -530,7 +533,6 @@ __global__ __launch_bounds__(64 * WAVES) void k_window_sort(
     return;
   }
   if (tid <= kWinH) s_rp[tid] = rowptr[min((int64_t)w * kWinH + tid, (int64_t)M)] - s;
-  if (tid == 0) s_unsorted = 0;
   __syncthreads();
 
   // key[k]: column - window minimum.  ix[k]: local index (low 16 bits)
@@ -550,24 +552,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_window_sort(
     cmin = q < E ? min(cmin, c) : cmin;
     cmax = q < E ? max(cmax, c) : cmax;
   }
-  // edgeToRow; and whether every row of the window is strictly increasing
-  // in column (sorted, no duplicates), which lets the TC-block positions
-  // below come from a row-major occupancy bitmap of each block
-  bool unsorted = false;
 #pragma unroll
   for (int k = 0; k < ITEMS; ++k) {
     const int q = vbase + k * 64 + lane;
-    const int up = __shfl_up((int)key[k], 1, 64);  // column of entry q - 1 (lanes 1..63)
-    if (q < E) {
-      const int r = row_of(s_rp, q);
-      e2r[s + q] = w * kWinH + r;
-      if (q > s_rp[r]) {  // not the first entry of its row
-        const int prev = lane > 0 ? up : col[s + q - 1];
-        unsorted |= (int)key[k] <= prev;
-      }
-    }
+    if (q < E) e2r[s + q] = w * kWinH + row_of(s_rp, q);
   }
-  if (__ballot(unsorted) != 0 && lane == 0) s_unsorted = 1;
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) {
     cmin = min(cmin, __shfl_xor(cmin, d, 64));
@@ -738,8 +727,6 @@ __global__ __launch_bounds__(64 * WAVES) void k_window_sort(
     }
   }
   if (lane == 0) s_wtot[v] = run;
-  // the counters are free now: they become the blocks' occupancy bitmaps
-  for (int i = tid; i < win_cnt(WAVES); i += NT) s_cnt[i] = 0;
   __syncthreads();
   int pre = 0, U = 0;
 #pragma unroll
@@ -750,10 +737,30 @@ __global__ __launch_bounds__(64 * WAVES) void k_window_sort(
   }
   const int nb = (U + kWinW - 1) / kWinW;
   // Bitmap path: block kb's 16 x 8 (row, slot) occupancy in words
-  // s_cnt[4 kb .. 4 kb + 3].  With strictly increasing rows, the CSR order of
-  // a block's entries is the row-major (row, slot) order, so an entry's
-  // place in its block is the number of set bits before its own.
-  const bool bitmap = s_unsorted == 0 && nb * 4 <= win_cnt(WAVES);
+  // s_cnt[4 kb .. 4 kb + 3] (the sort's counters are free now).  With
+  // strictly increasing rows, the CSR order of a block's entries is the
+  // row-major (row, slot) order, so an entry's place in its block is the
+  // number of set bits before its own.  It pays where blocks are full (at
+  // least 16 entries on average); otherwise each entry counts the few
+  // entries of its block with a smaller CSR index.
+  bool bitmap = false;
+  if (kBitmap && nb * 4 <= win_cnt(WAVES) && E >= 16 * nb) {  // workgroup-uniform
+    for (int i = tid; i < nb * 4; i += NT) s_cnt[i] = 0;
+    if (tid == 0) s_unsorted = 0;
+    __syncthreads();
+    bool unsorted = false;
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      const int q = vbase + k * 64 + lane;
+      if (q < E) {
+        const int r = row_of(s_rp, q);
+        if (q > s_rp[r]) unsorted |= col[s + q] <= col[s + q - 1];  // not a row's first entry
+      }
+    }
+    if (__ballot(unsorted) != 0 && lane == 0) s_unsorted = 1;
+    __syncthreads();
+    bitmap = s_unsorted == 0;
+  }
   if (bitmap) {
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {

@@ -114,6 +114,36 @@ def test_unsorted_rows_and_duplicates(ext, path):
     check(ext, rowptr, col, 1000, ctx="unsorted+dups")
 
 
+@pytest.mark.parametrize("kind", ["sorted", "unsorted", "dups", "one_swap"])
+def test_dense_large_windows(ext, path, kind):
+    """Windows of 5,120 to 14,400 entries over at most 1,200 columns (TC
+    blocks of 34 or more entries on average), for the 8- and 16-wave LDS
+    shapes: their TC-block positions come from per-block occupancy bitmaps
+    when every row of the window is strictly increasing, and from per-entry
+    counting otherwise (unsorted rows, duplicates, or one out-of-order pair
+    in a single row)."""
+    rng = np.random.default_rng(31 + len(kind))
+    rows = []
+    for w in range(6):
+        per_row = [320, 600, 900][w % 3]
+        pool = rng.choice(1 << 18, 1200, replace=False)
+        for _ in range(16):
+            rows.append(np.sort(rng.choice(pool, per_row, replace=False)))
+    if kind == "unsorted":
+        rows = [rng.permutation(r) for r in rows]
+    elif kind == "dups":
+        rows[5] = np.sort(np.concatenate([rows[5], rows[5][:3]]))
+        rows[16 * 4 + 9] = np.sort(np.concatenate([rows[16 * 4 + 9], rows[16 * 4 + 9][-1:]]))
+    elif kind == "one_swap":
+        for r in (16 + 7, 16 * 4 + 15):
+            rows[r] = rows[r].copy()
+            rows[r][[-2, -1]] = rows[r][[-1, -2]]
+    M = len(rows)
+    rowptr = np.concatenate([[0], np.cumsum([r.shape[0] for r in rows])]).astype(np.int32)
+    col = np.concatenate(rows).astype(np.int32)
+    check(ext, rowptr, col, M, ctx=f"dense large windows {kind}")
+
+
 def test_empty_and_degenerate(ext, path):
     for M in [0, 1, 15, 16, 17, 100]:
         rowptr = np.zeros(M + 1, np.int32)
