@@ -58,6 +58,8 @@ constexpr int kClassWaves[kClasses] = {1, 1, 2, 4, 8, 16};
 constexpr int kClassItems[kClasses] = {1, 8, 16, 16, 16, 16};
 constexpr int kClassCap[kClasses] = {64, 64 * 1 * 8, 64 * 2 * 16, 64 * 4 * 16, 64 * 8 * 16,
                                      64 * 16 * 16};
+// Phase 2 takes the TC blocks in chunks of kOutBlocks per workgroup.
+constexpr int kOutBlocks = 512;
 
 // Routing of windows (environment DTC_METCF_PATH): by size, every window
 // through the global path, or every window that fits class c through c.
@@ -102,6 +104,9 @@ struct MetcfWs {
   // 32-bit views of keys0 / keys1 for the segmented global-path sort
   uint32_t *k32_0, *k32_1;
   int32_t *wpos;            // global path: compacted window of each compacted position
+  int32_t *cw;              // [2][nnz / kOutBlocks + 2] window of the first and of the
+                            // last block of each phase-2 chunk
+  int64_t cw_stride;
   void *tmp;                // scan / sort scratch
   size_t bytes;
 };
@@ -129,6 +134,8 @@ MetcfWs carve(void *ws, int64_t nnz, int64_t W) {
   w.k32_0 = reinterpret_cast<uint32_t *>(w.keys0);
   w.k32_1 = reinterpret_cast<uint32_t *>(w.keys1);
   w.wpos = c.take<int32_t>(n);
+  w.cw_stride = ceil_div((int64_t)n, kOutBlocks) + 1;
+  w.cw = c.take<int32_t>((size_t)(2 * w.cw_stride));
   const size_t tmp = std::max({seg_radix_sort_workspace_bytes(nnz, W), scan_workspace_bytes(nnz),
                                scan_workspace_bytes(W)});
   w.tmp = c.take<char>(tmp);
@@ -931,40 +938,60 @@ __global__ __launch_bounds__(256) void k_window_tiny(
 }
 
 // Phase 2 (every window, both paths): a workgroup takes kOutBlocks
-// consecutive TC blocks.  Two waves find the windows of its first and last
-// block (rwo[w] <= b < rwo[w+1]) by cooperative searches; the row-window
-// offsets in between go to LDS (when they fit), where each block finds its
-// window.  Each block's offset and row id, and then SparseAToX (the window's
+// consecutive TC blocks.  The windows of its first and last block come from
+// k_chunk_windows; the row-window offsets and CSR starts of the windows in
+// between go to LDS (when they fit), where each block finds its window
+// (rwo[w] <= b < rwo[w+1]).  Each block's offset and row id, and then SparseAToX (the window's
 // distinct columns from kb * blk_w on, zero past U, consecutive threads on
 // consecutive entries) come from the WinStage data.
-constexpr int kOutBlocks = 512;
 constexpr int kOutWin = 1024;  // row-window offsets kept in LDS
+
+// Phase 1, after RowWindowOffset: the window of the first and of the last
+// block of every phase-2 chunk (blocks [c kOutBlocks, min((c + 1) kOutBlocks,
+// T))), so that its workgroups start without a search.  One thread per
+// window; a window writes the chunks whose first or last block it holds
+// (every block has at least one entry, so T <= nnz and c < cw_stride).
+__global__ __launch_bounds__(256) void k_chunk_windows(const int32_t *__restrict__ rwo, int32_t W,
+                                                       int32_t *__restrict__ cw_first,
+                                                       int32_t *__restrict__ cw_last) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  const int64_t b0 = rwo[w], b1 = rwo[w + 1];
+  if (b0 >= b1) return;  // empty window: no block
+  const int64_t T = rwo[W];
+  for (int64_t c = (b0 + kOutBlocks - 1) / kOutBlocks; c * kOutBlocks < b1; ++c)
+    cw_first[c] = w;
+  for (int64_t c = b0 / kOutBlocks; c <= (b1 - 1) / kOutBlocks; ++c) {
+    const int64_t last = std::min(c * kOutBlocks + kOutBlocks, T) - 1;
+    if (last >= b0 && last < b1) cw_last[c] = w;
+  }
+}
 __global__ __launch_bounds__(256) void k_blocks_out(const int32_t *__restrict__ rowptr, int32_t M,
                                                     int32_t W, int32_t blk_h, int32_t blk_w,
                                                     int32_t T, int32_t nnz, WinStage st,
                                                     const int32_t *__restrict__ rwo,
+                                                    const int32_t *__restrict__ cw_first,
+                                                    const int32_t *__restrict__ cw_last,
                                                     int32_t *__restrict__ off,
                                                     int32_t *__restrict__ rowid,
                                                     int32_t *__restrict__ atox) {
-  __shared__ int s_rwo[kOutWin + 1];
-  __shared__ int s_ends[2];
+  __shared__ int s_rwo[kOutWin + 1], s_ws[kOutWin + 1];
   __shared__ int s_base[kOutBlocks], s_kb[kOutBlocks], s_U[kOutBlocks];
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int tid = threadIdx.x;
   const int first = blockIdx.x * kOutBlocks;
   if (first >= T) return;  // workgroup-uniform
   if (first == 0 && tid == 0) off[T] = nnz;
   const int nblk = min(kOutBlocks, T - first);
-  if (wave < 2) {
-    const int b = wave ? first + nblk - 1 : first;
-    // window of block b: the last w with rwo[w] <= b
-    const int w = wave_lower_bound(rwo, 0, W + 1, b + 1) - 1;
-    if (lane == 0) s_ends[wave] = w;
-  }
-  __syncthreads();
-  const int wa = s_ends[0], wb = s_ends[1];
+  // (clamped into [0, W) so that the LDS range stays in bounds whatever the
+  // workspace holds)
+  const int wa = min(max(cw_first[blockIdx.x], 0), W - 1);
+  const int wb = min(max(cw_last[blockIdx.x], wa), W - 1);
   const bool in_lds = wb - wa + 1 <= kOutWin;
   if (in_lds)
-    for (int k = tid; k <= wb - wa + 1; k += blockDim.x) s_rwo[k] = rwo[wa + k];
+    for (int k = tid; k <= wb - wa + 1; k += blockDim.x) {
+      s_rwo[k] = rwo[wa + k];
+      s_ws[k] = rowptr[min((int64_t)(wa + k) * blk_h, (int64_t)M)];
+    }
   __syncthreads();
 #pragma unroll 2
   for (int t = tid; t < nblk; t += blockDim.x) {
@@ -981,7 +1008,7 @@ __global__ __launch_bounds__(256) void k_blocks_out(const int32_t *__restrict__ 
     } else {
       w = lower_bound_dev(rwo, wa + 1, wb + 1, b + 1) - 1;
     }
-    const int s = rowptr[min((int64_t)w * blk_h, (int64_t)M)];
+    const int s = in_lds ? s_ws[w - wa] : rowptr[min((int64_t)w * blk_h, (int64_t)M)];
     const int kb = b - (in_lds ? s_rwo[w - wa] : rwo[w]);
     off[b] = s + st.boff[s + kb];
     rowid[b] = w;
@@ -992,10 +1019,27 @@ __global__ __launch_bounds__(256) void k_blocks_out(const int32_t *__restrict__ 
   __syncthreads();
   const int n_atox = nblk * blk_w;
   const int64_t a0 = (int64_t)first * blk_w;
+  if (blk_w == kWinW && ((uintptr_t)atox & 15) == 0) {
+    // 8-column blocks: four consecutive entries (half a block) per thread and
+    // one 16-byte store (a0 is a multiple of 8)
+#pragma unroll 2
+    for (int e = tid * 4; e < n_atox; e += blockDim.x * 4) {
+      const int t = e >> 3, c = e & 7;
+      const int left = s_U[t] - (s_kb[t] + c);  // distinct columns left from this entry on
+      const int32_t *d = st.dcol + s_base[t] + c;
+      int4 v;
+      v.x = left > 0 ? d[0] : 0;
+      v.y = left > 1 ? d[1] : 0;
+      v.z = left > 2 ? d[2] : 0;
+      v.w = left > 3 ? d[3] : 0;
+      *reinterpret_cast<int4 *>(atox + a0 + e) = v;
+    }
+  } else {
 #pragma unroll 4
-  for (int e = tid; e < n_atox; e += blockDim.x) {
-    const int t = e / blk_w, c = e - t * blk_w;
-    atox[a0 + e] = s_kb[t] + c < s_U[t] ? st.dcol[s_base[t] + c] : 0;
+    for (int e = tid; e < n_atox; e += blockDim.x) {
+      const int t = e / blk_w, c = e - t * blk_w;
+      atox[a0 + e] = s_kb[t] + c < s_U[t] ? st.dcol[s_base[t] + c] : 0;
+    }
   }
 }
 
@@ -1519,6 +1563,9 @@ dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *no
   // 6. Window offsets (row_window_offset[0] was written by k_classify).
   st = scan_i32(block_partition, row_window_offset + 1, W, true, ws.tmp, stream);
   if (st != DTC_SUCCESS) return st;
+  k_chunk_windows<<<grid_for(W, 256), 256, 0, stream>>>(row_window_offset, (int32_t)W, ws.cw,
+                                                         ws.cw + ws.cw_stride);
+  DTC_LAUNCH_CHECK("chunk_windows");
 
   // 7. Block count (second host synchronisation).
   int32_t T = 0;
@@ -1550,6 +1597,8 @@ dtc_status_t dtc_metcf_build_blocks(const int32_t *node_pointer,
   dtc_status_t st = check_shape(num_nodes, nnz, blk_h, blk_w);
   if (st != DTC_SUCCESS) return st;
   if (block_count < 0) return fail(DTC_ERROR_INVALID_VALUE, "block_count must be >= 0");
+  if (block_count > nnz)  // every block holds an entry: phase 1 never reports more
+    return fail(DTC_ERROR_INVALID_VALUE, "block_count exceeds nnz: pass the count phase 1 returned");
   if (!node_pointer || !row_window_offset || !tcblock_offset)
     return fail(DTC_ERROR_INVALID_VALUE, "node_pointer, row_window_offset and tcblock_offset are required");
   if (block_count > 0 && (!tcblock_rowid || !sparse_atox))
@@ -1567,7 +1616,7 @@ dtc_status_t dtc_metcf_build_blocks(const int32_t *node_pointer,
   }
   k_blocks_out<<<grid_for(block_count, kOutBlocks), 256, 0, stream>>>(
       node_pointer, num_nodes, (int32_t)W, blk_h, blk_w, block_count, nnz, win_stage(ws),
-      row_window_offset, tcblock_offset, tcblock_rowid, sparse_atox);
+      row_window_offset, ws.cw, ws.cw + ws.cw_stride, tcblock_offset, tcblock_rowid, sparse_atox);
   DTC_LAUNCH_CHECK("blocks_out");
   return DTC_SUCCESS;
 }

@@ -126,6 +126,10 @@ def test_host_side_argument_errors(lib):
                                      dummy, None, None, ctypes.byref(ctypes.c_int32()), None,
                                      ctypes.c_size_t(0), None)
     assert st == 3
+    # phase 2 with a block count no phase 1 can return (more blocks than entries)
+    st = lib.dtc_metcf_build_blocks(dummy, dummy, 16, 4, 16, 8, 5, dummy, dummy, dummy, dummy,
+                                    ctypes.c_size_t(1 << 20), None)
+    assert st == 1 and b"block_count" in lib.dtc_last_error()
     # balanced launches without their workspace (persistent or not), or with
     # one smaller than the size query reports
     size = ctypes.c_size_t()

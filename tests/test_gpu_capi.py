@@ -62,7 +62,13 @@ def test_capi_metcf_and_spmm():
         atox = torch.empty(8 * T, dtype=i32, device="cuda")
         lib.metcf_build_blocks(rp_d.data_ptr(), rwo.data_ptr(), M, nnz, T, rowid.data_ptr(), off.data_ptr(),
                                atox.data_ptr(), ws.data_ptr(), ws_bytes, stream=s)
+        # phase 2 again into a SparseAToX 4 bytes past a 16-byte boundary (the
+        # build takes any int32-aligned buffer; the SpMMs need 16 bytes)
+        atox_odd = torch.full((8 * T + 1,), -7, dtype=i32, device="cuda")
+        lib.metcf_build_blocks(rp_d.data_ptr(), rwo.data_ptr(), M, nnz, T, rowid.data_ptr(), off.data_ptr(),
+                               atox_odd.data_ptr() + 4, ws.data_ptr(), ws_bytes, stream=s)
         torch.cuda.synchronize()
+        assert atox_odd[0].item() == -7 and torch.equal(atox_odd[1:], atox), name
         ref = oracle.metcf(rowptr, col, M, values=vals)
         assert T == ref["block_count"], name
         for k, v in [("RowWindowOffset", rwo), ("TCblockRowid", rowid), ("TCblocktileId", tile),
