@@ -174,8 +174,8 @@ __global__ void k_header_init(MetcfHeader *h) {
 // Checks rowptr[0] == 0, rowptr[M] == nnz, rowptr non-decreasing, columns
 // >= 0, and finds the largest column.  Columns are read 16 bytes per lane
 // when the array is 16-byte aligned.
-__global__ __launch_bounds__(256) void k_validate(const int32_t *rowptr, const int32_t *col,
-                                                  int32_t M, int32_t nnz, MetcfHeader *h) {
+__device__ __forceinline__ void validate_part(const int32_t *rowptr, const int32_t *col,
+                                              int32_t M, int32_t nnz, MetcfHeader *h) {
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   int flags = 0;
@@ -241,9 +241,9 @@ __device__ __forceinline__ int window_end(const int32_t *rowptr, int w, int blk_
 // kClasses, whose entries are summed into nnz_big).  Empty windows must have
 // blockPartition 0 (the caller zero-fills it; empty windows are not written).
 // Also writes row_window_offset[0] = 0.
-__global__ __launch_bounds__(256) void k_classify(const int32_t *rowptr, int32_t M, int32_t W,
-                                                  int32_t blk_h, int mode, const int32_t *bp,
-                                                  int32_t *lists, int32_t *rwo, MetcfHeader *h) {
+__device__ __forceinline__ void classify_part(const int32_t *rowptr, int32_t M, int32_t W,
+                                              int32_t blk_h, int mode, const int32_t *bp,
+                                              int32_t *lists, int32_t *rwo, MetcfHeader *h) {
   constexpr int NC = kClasses + 1;
   __shared__ int s_cnt[NC], s_base[NC], s_big;
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
@@ -297,6 +297,18 @@ __global__ __launch_bounds__(256) void k_classify(const int32_t *rowptr, int32_t
       lists[(int64_t)c * W + s_base[c] + woff[c] + below] = w;
     }
   }
+}
+
+// The CSR check and the window classes in one launch: every workgroup takes
+// its share of the check (a grid-stride walk over rows and edges) and classes
+// the 256 windows of its own index (none past W).
+__global__ __launch_bounds__(256) void k_check_classify(const int32_t *rowptr, const int32_t *col,
+                                                        int32_t M, int32_t nnz, int32_t W,
+                                                        int32_t blk_h, int mode, const int32_t *bp,
+                                                        int32_t *lists, int32_t *rwo,
+                                                        MetcfHeader *h) {
+  validate_part(rowptr, col, M, nnz, h);
+  classify_part(rowptr, M, W, blk_h, mode, bp, lists, rwo, h);
 }
 
 // Exclusive prefix sum by one 1,024-thread workgroup, 16 items per thread
@@ -353,23 +365,51 @@ __global__ __launch_bounds__(1024) void k_big_offsets(const int32_t *rowptr, int
                                                       int32_t blk_h, const int32_t *wmap,
                                                       const MetcfHeader *h, int32_t tile,
                                                       int32_t *csb, int32_t *tb, int32_t *tw) {
-  __shared__ int s_tmp[16];
+  constexpr int IT = kOneGroupWindows / 1024;  // windows per thread
+  __shared__ int s_e[16], s_t[16];
   const int n_big = h->cls_count[kClasses];
   if (n_big == 0 || n_big > kOneGroupWindows) return;  // workgroup-uniform
-  auto entries = [&](int i) {
-    const int w = wmap[i];
-    return window_end(rowptr, w, blk_h, M) - window_begin(rowptr, w, blk_h, M);
-  };
-  const int total = wg1024_exclusive_scan(entries, [&](int i, int x) { csb[i] = x; }, n_big, s_tmp);
-  const int tiles = wg1024_exclusive_scan([&](int i) { return (entries(i) + tile - 1) / tile; },
-                                          [&](int i, int x) { tb[i] = x; }, n_big, s_tmp);
-  if (threadIdx.x == 0) {
-    csb[n_big] = total;
-    tb[n_big] = tiles;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  // entries and sort tiles of this thread's windows, loaded once
+  int e[IT], t[IT], se = 0, st = 0;
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int i = tid * IT + it;
+    const int w = i < n_big ? wmap[i] : 0;
+    e[it] = i < n_big ? window_end(rowptr, w, blk_h, M) - window_begin(rowptr, w, blk_h, M) : 0;
+    t[it] = (e[it] + tile - 1) / tile;
+    se += e[it];
+    st += t[it];
+  }
+  const int ie = wave_inclusive_sum(se), itl = wave_inclusive_sum(st);
+  if (lane == 63) {
+    s_e[wave] = ie;
+    s_t[wave] = itl;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < n_big; i += blockDim.x)
-    for (int t = tb[i]; t < tb[i + 1]; ++t) tw[t] = i;
+  int re = ie - se, rt = itl - st, te = 0, tt = 0;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    re += j < wave ? s_e[j] : 0;
+    rt += j < wave ? s_t[j] : 0;
+    te += s_e[j];
+    tt += s_t[j];
+  }
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int i = tid * IT + it;
+    if (i < n_big) {
+      csb[i] = re;
+      tb[i] = rt;
+      for (int k = 0; k < t[it]; ++k) tw[rt + k] = i;
+    }
+    re += e[it];
+    rt += t[it];
+  }
+  if (tid == 0) {
+    csb[n_big] = te;
+    tb[n_big] = tt;
+  }
 }
 
 // Many global-path windows: the same offsets with one thread per window
@@ -1383,17 +1423,17 @@ dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *no
   k_header_init<<<1, 64, 0, stream>>>(ws.hdr);
   DTC_LAUNCH_CHECK("header_init");
   {
-    // few workgroups: each adds one atomic on the same header words
+    // few workgroups for the check (each adds one atomic on the same header
+    // words), and one per 256 windows for the classes
     const int64_t work = std::max<int64_t>(num_nodes, nnz);
-    const unsigned grid = (unsigned)std::min<int64_t>(1024, grid_for(work, 256));
-    k_validate<<<grid, 256, 0, stream>>>(node_pointer, edge_list, num_nodes, nnz, ws.hdr);
-    DTC_LAUNCH_CHECK("validate");
+    const unsigned grid = (unsigned)std::max<int64_t>(std::min<int64_t>(1024, grid_for(work, 256)),
+                                                      grid_for(W, 256));
+    k_check_classify<<<grid, 256, 0, stream>>>(node_pointer, edge_list, num_nodes, nnz, (int32_t)W,
+                                               blk_h, mode, block_partition, ws.lists,
+                                               row_window_offset, ws.hdr);
+    DTC_LAUNCH_CHECK("check_classify");
   }
   if (W > 0) {
-    k_classify<<<grid_for(W, 256), 256, 0, stream>>>(node_pointer, num_nodes, (int32_t)W, blk_h,
-                                                      mode, block_partition, ws.lists,
-                                                      row_window_offset, ws.hdr);
-    DTC_LAUNCH_CHECK("classify");
     // offsets and sort tiles of up to kOneGroupWindows global-path windows
     k_big_offsets<<<1, 1024, 0, stream>>>(node_pointer, num_nodes, blk_h,
                                           ws.lists + (int64_t)kClasses * W, ws.hdr,
@@ -1403,6 +1443,15 @@ dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *no
   SideStream *dev_state = nullptr;
   st = side_stream(&dev_state);
   if (st != DTC_SUCCESS) return st;
+  // The side stream is forked here, before the first host synchronisation,
+  // so that the global path's first kernel starts as soon as it is issued;
+  // the lock keeps other builds on this device off the side stream until
+  // the join.
+  std::unique_lock<std::mutex> side_lock(dev_state->mu);
+  if (W > 0) {
+    DTC_HIP_CHECK(hipEventRecord(dev_state->fork, stream));
+    DTC_HIP_CHECK(hipStreamWaitEvent(dev_state->s, dev_state->fork, 0));
+  }
   MetcfHeader host{};
   {
     std::lock_guard<std::mutex> g(dev_state->read_mu);
@@ -1435,21 +1484,20 @@ dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *no
   const int nnz_big = host.nnz_big;
   const int32_t *wmap = ws.lists + (int64_t)kClasses * W;
 
-  // 2. Fork: the global path goes to the side stream when the LDS path also
-  //    has work.  Issue order: the global path's first kernel, the few long
-  //    LDS launches, then the global path's many short ones, so that both
-  //    streams start at once (the global path is the longer of the two).
+  // 2. The global path goes to the side stream when the LDS path also has
+  //    work.  Issue order: the global path's keys, the largest LDS class
+  //    (the longest kernel), the rest of the global path (many short
+  //    kernels), then the other LDS classes, so that neither stream waits
+  //    for the host's launches on the other.
   int n_lds = 0;
   for (int c = 0; c < kClasses; ++c) n_lds += host.cls_count[c];
   SideStream *side = nullptr;
-  std::unique_lock<std::mutex> side_lock;
   hipStream_t gs = stream;
   if (nnz_big > 0 && n_lds > 0) {
     side = dev_state;
-    side_lock = std::unique_lock<std::mutex>(side->mu);
     gs = side->s;
-    DTC_HIP_CHECK(hipEventRecord(side->fork, stream));
-    DTC_HIP_CHECK(hipStreamWaitEvent(gs, side->fork, 0));
+  } else {
+    side_lock.unlock();
   }
   // 3. Global path, first steps (issued before the LDS path so that the
   //    side stream starts at once): window offsets and sort tiles of many
@@ -1484,28 +1532,28 @@ dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *no
     DTC_LAUNCH_CHECK("big_keys");
   }
 
-  // 4. LDS path, largest class first: every output of the window sized by
-  //    its entries, and its block data for phase 2.
-  for (int c = kClasses - 1; c >= 0 && st == DTC_SUCCESS; --c) {
+  // 4. LDS path (every output of the window sized by its entries, and its
+  //    block data for phase 2): the largest class now, the others after the
+  //    global path's launches.
+  auto launch_class = [&](int c) -> dtc_status_t {
     const int n = host.cls_count[c];
-    if (c == 0) {
-      if (n > 0) {
+    if (n <= 0) return DTC_SUCCESS;
+    switch (c) {
+      case 0:
         k_window_tiny<<<grid_for(n, 4), 256, 0, stream>>>(
             node_pointer, edge_list, num_nodes, ws.lists, n, values, edge_to_row,
             edge_to_column, block_partition, tcblocktile_id, tcblock_values, tcblock_edge,
             win_stage(ws), ws.hdr);
         DTC_LAUNCH_CHECK("window_tiny");
-      }
-      continue;
+        return DTC_SUCCESS;
+      case 5: return launch_window_sort<5>(node_pointer, edge_list, num_nodes, n, ws, W, values, edge_to_row, edge_to_column, block_partition, tcblocktile_id, tcblock_values, tcblock_edge, stream);
+      case 4: return launch_window_sort<4>(node_pointer, edge_list, num_nodes, n, ws, W, values, edge_to_row, edge_to_column, block_partition, tcblocktile_id, tcblock_values, tcblock_edge, stream);
+      case 3: return launch_window_sort<3>(node_pointer, edge_list, num_nodes, n, ws, W, values, edge_to_row, edge_to_column, block_partition, tcblocktile_id, tcblock_values, tcblock_edge, stream);
+      case 2: return launch_window_sort<2>(node_pointer, edge_list, num_nodes, n, ws, W, values, edge_to_row, edge_to_column, block_partition, tcblocktile_id, tcblock_values, tcblock_edge, stream);
+      default: return launch_window_sort<1>(node_pointer, edge_list, num_nodes, n, ws, W, values, edge_to_row, edge_to_column, block_partition, tcblocktile_id, tcblock_values, tcblock_edge, stream);
     }
-    switch (c) {
-      case 5: st = launch_window_sort<5>(node_pointer, edge_list, num_nodes, n, ws, W, values, edge_to_row, edge_to_column, block_partition, tcblocktile_id, tcblock_values, tcblock_edge, stream); break;
-      case 4: st = launch_window_sort<4>(node_pointer, edge_list, num_nodes, n, ws, W, values, edge_to_row, edge_to_column, block_partition, tcblocktile_id, tcblock_values, tcblock_edge, stream); break;
-      case 3: st = launch_window_sort<3>(node_pointer, edge_list, num_nodes, n, ws, W, values, edge_to_row, edge_to_column, block_partition, tcblocktile_id, tcblock_values, tcblock_edge, stream); break;
-      case 2: st = launch_window_sort<2>(node_pointer, edge_list, num_nodes, n, ws, W, values, edge_to_row, edge_to_column, block_partition, tcblocktile_id, tcblock_values, tcblock_edge, stream); break;
-      default: st = launch_window_sort<1>(node_pointer, edge_list, num_nodes, n, ws, W, values, edge_to_row, edge_to_column, block_partition, tcblocktile_id, tcblock_values, tcblock_edge, stream); break;
-    }
-  }
+  };
+  st = launch_class(kClasses - 1);
   if (st != DTC_SUCCESS) return st;
   // 5. Global path, continued: sort, unique-column numbering;
   //    blockPartition, edgeToColumn, block data, tile ids; then join.
@@ -1554,20 +1602,24 @@ dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *no
     DTC_LAUNCH_CHECK("big_tile_ids");
   }
   if (side) DTC_HIP_CHECK(hipEventRecord(side->join, gs));
-
+  // 6. The other LDS classes.
+  for (int c = kClasses - 2; c >= 0; --c) {
+    st = launch_class(c);
+    if (st != DTC_SUCCESS) return st;
+  }
   if (side) {
     DTC_HIP_CHECK(hipStreamWaitEvent(stream, side->join, 0));
     side_lock.unlock();
   }
 
-  // 6. Window offsets (row_window_offset[0] was written by k_classify).
+  // 7. Window offsets (row_window_offset[0] was written by k_check_classify).
   st = scan_i32(block_partition, row_window_offset + 1, W, true, ws.tmp, stream);
   if (st != DTC_SUCCESS) return st;
   k_chunk_windows<<<grid_for(W, 256), 256, 0, stream>>>(row_window_offset, (int32_t)W, ws.cw,
                                                          ws.cw + ws.cw_stride);
   DTC_LAUNCH_CHECK("chunk_windows");
 
-  // 7. Block count (second host synchronisation).
+  // 8. Block count (second host synchronisation).
   int32_t T = 0;
   {
     std::lock_guard<std::mutex> g(dev_state->read_mu);
