@@ -1206,7 +1206,7 @@ BigView big_view(const uint32_t *keys32, const uint64_t *keys64, int colbits, co
 __global__ __launch_bounds__(1024) void k_big_counts(BigView bv, int32_t n_big, int32_t blk_w,
                                                      int32_t *bp, int32_t *wU, int32_t *cbb) {
   __shared__ int s_tmp[16];
-  wg1024_exclusive_scan(
+  const int total = wg1024_exclusive_scan(
       [&](int wp) {
         const int c0 = bv.csb[wp], c1 = bv.csb[wp + 1];
         const int U = bv.g[c1 - 1] - (c0 > 0 ? bv.g[c0 - 1] : 0);
@@ -1216,6 +1216,7 @@ __global__ __launch_bounds__(1024) void k_big_counts(BigView bv, int32_t n_big, 
         return nb;
       },
       [&](int wp, int x) { cbb[wp] = x; }, n_big, s_tmp);
+  if (threadIdx.x == 0) cbb[n_big] = total;  // as the multi-workgroup path leaves it
 }
 
 // The same with one thread per window (cbb then holds the block counts, with
