@@ -54,9 +54,11 @@ constexpr int kWinW = 8;   // columns per TC block on the LDS path (BLK_W)
 // Class 0 (at most 64 entries) is one wave per window with bitonic sorts
 // (k_window_tiny); classes 1-5 are k_window_sort.
 constexpr int kClasses = 6;
-constexpr int kClassWaves[kClasses] = {1, 1, 2, 4, 8, 16};
-constexpr int kClassItems[kClasses] = {1, 8, 16, 16, 16, 16};
-constexpr int kClassCap[kClasses] = {64, 64 * 1 * 8, 64 * 2 * 16, 64 * 4 * 16, 64 * 8 * 16,
+// (Measured against 2 x 16 and 4 x 16 for the middle classes, 2 x 4 for
+// class 1 and 16 x 8 for class 4 on power-law and block-model matrices.)
+constexpr int kClassWaves[kClasses] = {1, 1, 4, 8, 8, 16};
+constexpr int kClassItems[kClasses] = {1, 8, 8, 8, 16, 16};
+constexpr int kClassCap[kClasses] = {64, 64 * 1 * 8, 64 * 4 * 8, 64 * 8 * 8, 64 * 8 * 16,
                                      64 * 16 * 16};
 // Phase 2 takes the TC blocks in chunks of kOutBlocks per workgroup.
 constexpr int kOutBlocks = 512;
@@ -510,11 +512,11 @@ __device__ __forceinline__ void counts_to_offsets(int *cnt, int *s_scan) {
 // Digit widths of the window sort per workgroup shape: the first pass (not
 // stable, one workgroup-wide counter row) takes D1 bits, the later ones
 // (stable, one counter row per wave) D2 bits, so that a 20-bit column span
-// takes two passes in the 2- to 16-wave shapes while the counters stay small
+// takes two passes in the 4- to 16-wave shapes while the counters stay small
 // enough for four waves per SIMD (the 1-wave shape, for windows of at most
 // 512 entries, keeps 8-bit digits: its fixed cost per digit dominates).  The
-// 1-, 2- and 4-wave shapes move key and index in one step through a 32-bit
-// and a 16-bit buffer; the larger ones through one 16-bit buffer.
+// shapes of at most 4,096 entries move key and index in one step through a
+// 32-bit and a 16-bit buffer; the larger ones through one 16-bit buffer.
 constexpr int win_d1(int waves) { return waves == 1 ? 8 : waves <= 4 ? 11 : 12; }
 constexpr int win_d2(int waves) { return waves == 1 ? 8 : waves <= 8 ? 9 : 8; }
 constexpr int win_cnt(int waves) {

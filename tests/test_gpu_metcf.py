@@ -4,8 +4,9 @@ Most tests run under every window routing of the build (DTC_METCF_PATH):
 "auto" (by entry count: one LDS workgroup per window up to 16,384 entries,
 the device-wide radix sort beyond), "global" (every window through the
 radix sort) and "lds0" ... "lds5" (every window that fits the one-wave
-bitonic kernel for at most 64 entries, or the 1-, 2-, 4-, 8- or 16-wave
-LDS radix-sort shape, through it; the rest through the radix sort)."""
+bitonic kernel for at most 64 entries, or one LDS radix-sort shape, waves x
+entries per lane 1 x 8, 4 x 8, 8 x 8, 8 x 16 or 16 x 16, through it; the
+rest through the radix sort)."""
 import os
 
 import numpy as np
