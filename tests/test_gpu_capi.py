@@ -128,3 +128,76 @@ def test_capi_errors_are_status_codes():
         lib.metcf_build_windows(col_d.data_ptr(), rp_d.data_ptr(), 3, 3, *[b.data_ptr() for b in bufs],
                                 tile.data_ptr(), ws.data_ptr(), ws_bytes - 1, stream=s)
     assert e.value.status == 3
+
+
+def _capi_build(lib, rowptr, col, M, stream):
+    """Phase 1 and phase 2 through the C ABI on `stream` (torch allocations
+    on the current stream); returns the ME-TCF arrays on the host."""
+    i32, u8 = torch.int32, torch.uint8
+    nnz = int(col.shape[0])
+    W = (M + 15) // 16
+    col_d, rp_d = _dev(col), _dev(rowptr)
+    ws_bytes = lib.metcf_workspace_size(M, nnz)
+    ws = torch.empty(ws_bytes, dtype=u8, device="cuda")
+    bp = torch.zeros(W, dtype=i32, device="cuda")
+    e2c = torch.empty(nnz, dtype=i32, device="cuda")
+    e2r = torch.empty(nnz, dtype=i32, device="cuda")
+    rwo = torch.empty(W + 1, dtype=i32, device="cuda")
+    tile = torch.empty(nnz, dtype=u8, device="cuda")
+    T = lib.metcf_build_windows(col_d.data_ptr(), rp_d.data_ptr(), M, nnz, bp.data_ptr(), e2c.data_ptr(),
+                                e2r.data_ptr(), rwo.data_ptr(), tile.data_ptr(), ws.data_ptr(), ws_bytes,
+                                stream=stream)
+    rowid = torch.empty(T, dtype=i32, device="cuda")
+    off = torch.empty(T + 1, dtype=i32, device="cuda")
+    atox = torch.empty(8 * T, dtype=i32, device="cuda")
+    lib.metcf_build_blocks(rp_d.data_ptr(), rwo.data_ptr(), M, nnz, T, rowid.data_ptr(), off.data_ptr(),
+                           atox.data_ptr(), ws.data_ptr(), ws_bytes, stream=stream)
+    torch.cuda.current_stream().synchronize()
+    return T, {"RowWindowOffset": rwo, "TCblockRowid": rowid, "TCblocktileId": tile,
+               "TCblockoffset": off, "SparseAToXindex": atox, "blockPartition": bp,
+               "edgeToColumn": e2c, "edgeToRow": e2r}
+
+
+def test_capi_metcf_concurrent_threads():
+    """Two host threads building ME-TCF at the same time through the C ABI
+    (ctypes releases the GIL), each on a stream of its own, on matrices with a
+    window of 48,000 entries (the device-wide sort, on the library's side
+    stream) beside thousands of small ones (the LDS path): the builds share
+    the side stream and the pinned read buffer, and every one of them must be
+    bit-exact against the oracle."""
+    import threading
+    from dtc.capi import Library
+    lib = Library()
+    rng = np.random.default_rng(5)
+    cases = []
+    for k in range(2):
+        rows = [np.sort(rng.choice(1 << 16, 3000, replace=False)) for _ in range(16)]
+        rows += [np.sort(rng.choice(1 << 16, int(rng.integers(1, 9)), replace=False))
+                 for _ in range(2000 + 16 * k)]
+        M = len(rows)
+        rowptr = np.concatenate([[0], np.cumsum([r.shape[0] for r in rows])]).astype(np.int32)
+        col = np.concatenate(rows).astype(np.int32)
+        cases.append((rowptr, col, M, oracle.metcf(rowptr, col, M)))
+    errors = []
+
+    def work(idx):
+        try:
+            rowptr, col, M, ref = cases[idx]
+            s = torch.cuda.Stream()
+            with torch.cuda.stream(s):
+                for rep in range(4):
+                    T, out = _capi_build(lib, rowptr, col, M, s.cuda_stream)
+                    assert T == ref["block_count"], (idx, rep)
+                    for k, v in out.items():
+                        assert np.array_equal(v.cpu().numpy(), ref[k]), (idx, rep, k)
+        except Exception as e:  # reported by the main thread
+            errors.append(f"thread {idx}: {e!r}")
+
+    say("capi", "two threads, four builds each")
+    threads = [threading.Thread(target=work, args=(i,)) for i in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    assert not any(t.is_alive() for t in threads), "a build thread did not finish"
+    assert not errors, errors
