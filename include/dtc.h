@@ -115,6 +115,9 @@ dtc_status_t dtc_metcf_workspace_size(int32_t num_nodes, int32_t nnz, int32_t bl
  * NULL); tcblock_edge [nnz] receives the CSR edge index stored at each
  * TC-block position.  Synchronises the stream twice (CSR check, then the
  * block count), as the reference does (DTCSpMM_kernel.cu:378-380, :384-387).
+ * Windows of more than 16,384 entries are sorted on a stream the library
+ * creates once per device (default priority), forked from and joined back to
+ * `stream` by events; builds on one device take turns at it under a lock.
  */
 dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *node_pointer,
                                      int32_t num_nodes, int32_t nnz, int32_t blk_h,
@@ -129,6 +132,8 @@ dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *no
 /*
  * Phase 2 (same workspace, after phase 1; no host synchronisation).  Out:
  * tcblock_rowid [T], tcblock_offset [T + 1], sparse_atox [T * blk_w].
+ * block_count must be the T phase 1 returned (a count above nnz is rejected:
+ * every block holds an entry).
  */
 dtc_status_t dtc_metcf_build_blocks(const int32_t *node_pointer,
                                     const int32_t *row_window_offset, int32_t num_nodes,
