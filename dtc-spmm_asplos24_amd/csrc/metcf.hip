@@ -1488,10 +1488,11 @@ dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *no
   const int32_t *wmap = ws.lists + (int64_t)kClasses * W;
 
   // 2. The global path goes to the side stream when the LDS path also has
-  //    work.  Issue order: the global path's keys, the largest LDS class
-  //    (the longest kernel), the rest of the global path (many short
-  //    kernels), then the other LDS classes, so that neither stream waits
-  //    for the host's launches on the other.
+  //    work.  Issue order: the global path's keys, the two largest LDS
+  //    classes (the longest kernels), the global path's sort, the third
+  //    class, the rest of the global path (many short kernels), then the
+  //    other LDS classes, so that neither stream waits for the host's
+  //    launches on the other.
   int n_lds = 0;
   for (int c = 0; c < kClasses; ++c) n_lds += host.cls_count[c];
   SideStream *side = nullptr;
@@ -1556,7 +1557,16 @@ dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *no
       default: return launch_window_sort<1>(node_pointer, edge_list, num_nodes, n, ws, W, values, edge_to_row, edge_to_column, block_partition, tcblocktile_id, tcblock_values, tcblock_edge, stream);
     }
   };
-  st = launch_class(kClasses - 1);
+  // The two largest classes now (a few hundred µs of work queued on the
+  // caller's stream while the host issues the global path), the third after
+  // the sort, the rest after the global path.
+  bool issued[kClasses] = {};
+  auto issue = [&](int c) -> dtc_status_t {
+    issued[c] = true;
+    return launch_class(c);
+  };
+  st = issue(kClasses - 1);
+  if (st == DTC_SUCCESS) st = issue(kClasses - 2);
   if (st != DTC_SUCCESS) return st;
   // 5. Global path, continued: sort, unique-column numbering;
   //    blockPartition, edgeToColumn, block data, tile ids; then join.
@@ -1584,6 +1594,10 @@ dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *no
       bv = big_view(nullptr, sorted, colbits, in_1 ? ws.vals1 : ws.vals0, ws.gscan, ws.csb, wmap,
                     ws.wpos);
     }
+    if (side) {
+      st = issue(kClasses - 3);
+      if (st != DTC_SUCCESS) return st;
+    }
     if (n_big <= kOneGroupWindows) {
       k_big_counts<<<1, 1024, 0, gs>>>(bv, n_big, blk_w, block_partition, ws.wU, ws.cbb);
       DTC_LAUNCH_CHECK("big_counts");
@@ -1606,8 +1620,9 @@ dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *no
   }
   if (side) DTC_HIP_CHECK(hipEventRecord(side->join, gs));
   // 6. The other LDS classes.
-  for (int c = kClasses - 2; c >= 0; --c) {
-    st = launch_class(c);
+  for (int c = kClasses - 1; c >= 0; --c) {
+    if (issued[c]) continue;
+    st = issue(c);
     if (st != DTC_SUCCESS) return st;
   }
   if (side) {
