@@ -173,61 +173,25 @@ __global__ void k_header_init(MetcfHeader *h) {
 }
 
 
-// Checks rowptr[0] == 0, rowptr[M] == nnz, rowptr non-decreasing, columns
-// >= 0, and finds the largest column.  Columns are read 16 bytes per lane
-// when the array is 16-byte aligned.
-__device__ __forceinline__ void validate_part(const int32_t *rowptr, const int32_t *col,
-                                              int32_t M, int32_t nnz, MetcfHeader *h) {
+// Checks rowptr[0] == 0, rowptr[M] == nnz and rowptr non-decreasing, which
+// keeps every window's entries inside the edge arrays.  (Columns are checked
+// where they are read: negative ones by the window kernels and by
+// k_big_maxcol, which also finds the largest column the global path sorts.)
+__device__ __forceinline__ void validate_part(const int32_t *rowptr, int32_t M, int32_t nnz,
+                                              MetcfHeader *h) {
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   int flags = 0;
-  int mx = 0;
   for (int64_t r = tid; r < M; r += stride)
     if (rowptr[r] > rowptr[r + 1]) flags |= kBadOrder;
-  const int64_t n4 = ((uintptr_t)col % 16 == 0) ? nnz / 4 : 0;
-  const int4 *col4 = reinterpret_cast<const int4 *>(col);
-  int neg = 0;
-  for (int64_t i = tid; i < n4; i += 4 * stride) {
-    int4 c[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int64_t j = i + u * stride;
-      c[u] = j < n4 ? col4[j] : make_int4(0, 0, 0, 0);
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      neg |= c[u].x | c[u].y | c[u].z | c[u].w;
-      mx = max(mx, max(max(c[u].x, c[u].y), max(c[u].z, c[u].w)));
-    }
-  }
-  for (int64_t e = n4 * 4 + tid; e < nnz; e += stride) {
-    const int c = col[e];
-    neg |= c;
-    mx = max(mx, c);
-  }
-  if (neg < 0) flags |= kBadColumn;
   if (tid == 0) {
     if (rowptr[0] != 0) flags |= kBadFirst;
     if (rowptr[M] != nnz) flags |= kBadLast;
   }
+  if (__ballot(flags != 0) == 0) return;  // wave-uniform: nothing to report
 #pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) {
-    flags |= __shfl_xor(flags, d, 64);
-    mx = max(mx, __shfl_xor(mx, d, 64));
-  }
-  // one atomic pair per workgroup
-  __shared__ int s_f[4], s_m[4];
-  if ((threadIdx.x & 63) == 0) {
-    s_f[threadIdx.x >> 6] = flags;
-    s_m[threadIdx.x >> 6] = mx;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const int bf = s_f[0] | s_f[1] | s_f[2] | s_f[3];
-    if (bf) atomicOr(&h->flags, bf);
-    // spread over kMaxSlots words: atomics on one word serialise in L2
-    atomicMax(&h->max_col[blockIdx.x % kMaxSlots], max(max(s_m[0], s_m[1]), max(s_m[2], s_m[3])));
-  }
+  for (int d = 32; d >= 1; d >>= 1) flags |= __shfl_xor(flags, d, 64);
+  if ((threadIdx.x & 63) == 0) atomicOr(&h->flags, flags);
 }
 
 __device__ __forceinline__ int window_begin(const int32_t *rowptr, int w, int blk_h, int M) {
@@ -301,15 +265,15 @@ __device__ __forceinline__ void classify_part(const int32_t *rowptr, int32_t M, 
   }
 }
 
-// The CSR check and the window classes in one launch: every workgroup takes
-// its share of the check (a grid-stride walk over rows and edges) and classes
-// the 256 windows of its own index (none past W).
-__global__ __launch_bounds__(256) void k_check_classify(const int32_t *rowptr, const int32_t *col,
-                                                        int32_t M, int32_t nnz, int32_t W,
-                                                        int32_t blk_h, int mode, const int32_t *bp,
+// The row-pointer check and the window classes in one launch: every
+// workgroup takes its share of the check (a grid-stride walk over the rows)
+// and classes the 256 windows of its own index (none past W).
+__global__ __launch_bounds__(256) void k_check_classify(const int32_t *rowptr, int32_t M,
+                                                        int32_t nnz, int32_t W, int32_t blk_h,
+                                                        int mode, const int32_t *bp,
                                                         int32_t *lists, int32_t *rwo,
                                                         MetcfHeader *h) {
-  validate_part(rowptr, col, M, nnz, h);
+  validate_part(rowptr, M, nnz, h);
   classify_part(rowptr, M, W, blk_h, mode, bp, lists, rwo, h);
 }
 
@@ -411,6 +375,75 @@ __global__ __launch_bounds__(1024) void k_big_offsets(const int32_t *rowptr, int
   if (tid == 0) {
     csb[n_big] = te;
     tb[n_big] = tt;
+  }
+}
+
+// Largest column of the global path's windows (the width of its sort keys)
+// and negative columns there, read by tiles of `tile` entries as laid out by
+// k_big_offsets; with more than kOneGroupWindows such windows (their tiles
+// are laid out after the first host read) every column is read.  One atomic
+// pair per workgroup.
+__global__ __launch_bounds__(256) void k_big_maxcol(const int32_t *rowptr, const int32_t *col,
+                                                    int32_t M, int32_t nnz, int32_t blk_h,
+                                                    const int32_t *wmap, const int32_t *csb,
+                                                    const int32_t *tb, const int32_t *tw,
+                                                    int32_t tile, MetcfHeader *h) {
+  const int n_big = h->cls_count[kClasses];
+  if (n_big == 0) return;  // workgroup-uniform
+  int mx = 0, neg = 0;
+  if (n_big <= kOneGroupWindows) {
+    const int tiles = tb[n_big];
+    for (int t = blockIdx.x; t < tiles; t += gridDim.x) {
+      const int k = tw[t];
+      const int s = window_begin(rowptr, wmap[k], blk_h, M);
+      const int off = (t - tb[k]) * tile;
+      const int len = min(tile, csb[k + 1] - csb[k] - off);
+      for (int i = threadIdx.x; i < len; i += blockDim.x) {
+        const int c = col[s + off + i];
+        mx = max(mx, c);
+        neg |= c;
+      }
+    }
+  } else {
+    // every column, 16 bytes per lane when the array is 16-byte aligned
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t n4 = ((uintptr_t)col % 16 == 0) ? nnz / 4 : 0;
+    const int4 *col4 = reinterpret_cast<const int4 *>(col);
+    for (int64_t i = tid; i < n4; i += 4 * stride) {
+      int4 c[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int64_t j = i + u * stride;
+        c[u] = j < n4 ? col4[j] : make_int4(0, 0, 0, 0);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        neg |= c[u].x | c[u].y | c[u].z | c[u].w;
+        mx = max(mx, max(max(c[u].x, c[u].y), max(c[u].z, c[u].w)));
+      }
+    }
+    for (int64_t e = n4 * 4 + tid; e < nnz; e += stride) {
+      const int c = col[e];
+      neg |= c;
+      mx = max(mx, c);
+    }
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    neg |= __shfl_xor(neg, d, 64);
+    mx = max(mx, __shfl_xor(mx, d, 64));
+  }
+  __shared__ int s_m[4], s_n[4];
+  if ((threadIdx.x & 63) == 0) {
+    s_m[threadIdx.x >> 6] = mx;
+    s_n[threadIdx.x >> 6] = neg;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if ((s_n[0] | s_n[1] | s_n[2] | s_n[3]) < 0) atomicOr(&h->flags, kBadColumn);
+    // spread over kMaxSlots words: atomics on one word serialise in L2
+    atomicMax(&h->max_col[blockIdx.x % kMaxSlots], max(max(s_m[0], s_m[1]), max(s_m[2], s_m[3])));
   }
 }
 
@@ -621,6 +654,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_window_sort(
     cmin = min(cmin, s_min[vv]);
     cmax = max(cmax, s_max[vv]);
   }
+  if (tid == 0 && cmin < 0) atomicOr(&h->flags, kBadColumn);  // reported after phase 1
   const uint32_t span = (uint32_t)(cmax - cmin);
   const int bits = span ? 32 - __clz(span) : 0;
 #pragma unroll
@@ -944,6 +978,7 @@ __global__ __launch_bounds__(256) void k_window_tiny(
     cmax = max(cmax, __shfl_xor(cmax, d, 64));
   }
   (void)cmax;
+  if (lane == 0 && cmin < 0) atomicOr(&h->flags, kBadColumn);  // reported after phase 1
   // sort 1: (column - window minimum, local index); empty lanes last
   uint64_t k1 = valid ? (((uint64_t)(uint32_t)(c - cmin) << 6) | (uint64_t)q) : ~0ull;
   k1 = wave_bitonic_sort(k1);
@@ -1421,18 +1456,18 @@ dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *no
     return fail(DTC_ERROR_WORKSPACE, "ME-TCF workspace needs " + std::to_string(ws.bytes) + " bytes");
   const int mode = path_mode(blk_h, blk_w);
 
-  // 1. CSR check, largest column and the window classes (first host
+  // 1. Row-pointer check, window classes, the global path's largest column
+  //    (negative columns are flagged where they are read) (first host
   //    synchronisation).
   k_header_init<<<1, 64, 0, stream>>>(ws.hdr);
   DTC_LAUNCH_CHECK("header_init");
   {
-    // few workgroups for the check (each adds one atomic on the same header
-    // words), and one per 256 windows for the classes
-    const int64_t work = std::max<int64_t>(num_nodes, nnz);
-    const unsigned grid = (unsigned)std::max<int64_t>(std::min<int64_t>(1024, grid_for(work, 256)),
-                                                      grid_for(W, 256));
-    k_check_classify<<<grid, 256, 0, stream>>>(node_pointer, edge_list, num_nodes, nnz, (int32_t)W,
-                                               blk_h, mode, block_partition, ws.lists,
+    // up to 1,024 workgroups for the row check, and one per 256 windows for
+    // the classes
+    const unsigned grid = (unsigned)std::max<int64_t>(
+        std::min<int64_t>(1024, grid_for(num_nodes, 256)), grid_for(W, 256));
+    k_check_classify<<<grid, 256, 0, stream>>>(node_pointer, num_nodes, nnz, (int32_t)W, blk_h,
+                                               mode, block_partition, ws.lists,
                                                row_window_offset, ws.hdr);
     DTC_LAUNCH_CHECK("check_classify");
   }
@@ -1442,6 +1477,10 @@ dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *no
                                           ws.lists + (int64_t)kClasses * W, ws.hdr,
                                           (int32_t)seg_radix_tile_size(), ws.csb, ws.tb, ws.tw);
     DTC_LAUNCH_CHECK("big_offsets");
+    k_big_maxcol<<<1024, 256, 0, stream>>>(node_pointer, edge_list, num_nodes, nnz, blk_h,
+                                           ws.lists + (int64_t)kClasses * W, ws.csb, ws.tb,
+                                           ws.tw, (int32_t)seg_radix_tile_size(), ws.hdr);
+    DTC_LAUNCH_CHECK("big_maxcol");
   }
   SideStream *dev_state = nullptr;
   st = side_stream(&dev_state);
@@ -1649,6 +1688,8 @@ dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *no
     std::memcpy(&host, dev_state->pinned, kHeaderRead);
     std::memcpy(&T, dev_state->pinned + kHeaderRead, sizeof(int32_t));
   }
+  if (host.flags & kBadColumn)  // found by the window kernels
+    return fail(DTC_ERROR_INVALID_CSR, "invalid CSR: negative column index;");
   if (host.flags & kBadClass)
     return fail(DTC_ERROR_INCONSISTENT, "internal error: a window exceeded its class capacity");
   *block_count = T;

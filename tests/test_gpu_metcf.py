@@ -175,6 +175,25 @@ def test_invalid_csr_raises(ext):
         gpu_metcf(ext, np.array([0, 2, 3], np.int32), np.array([0, 1, 2, 3], np.int32), 2)
 
 
+@pytest.mark.parametrize("where", ["tiny", "medium", "large"])
+def test_negative_column_every_path(ext, path, where):
+    """A negative column in a window of 48 entries, of 992 entries, or of
+    20,000 entries (the bitonic kernel, an LDS radix-sort shape, or the
+    device-wide sort under the automatic routing; the others under the
+    forced ones) is reported as an invalid CSR whichever kernel reads it."""
+    rng = np.random.default_rng(len(where))
+    sizes = [3] * 16 + [62] * 16 + [1250] * 16 + [5] * 16
+    rows = [np.sort(rng.choice(1 << 16, n, replace=False)) for n in sizes]
+    r = {"tiny": 3, "medium": 16 + 7, "large": 32 + 11}[where]
+    rows[r] = rows[r].copy()
+    rows[r][0] = -5
+    rowptr = np.concatenate([[0], np.cumsum([x.shape[0] for x in rows])]).astype(np.int32)
+    col = np.concatenate(rows).astype(np.int32)
+    say("metcf negative column", where, "path=" + path)
+    with pytest.raises(RuntimeError, match="invalid CSR"):
+        gpu_metcf(ext, rowptr, col, len(rows))
+
+
 def test_wrong_sizes_raise(ext):
     dev = "cuda"
     i32 = torch.int32
