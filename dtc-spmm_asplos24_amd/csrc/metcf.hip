@@ -163,6 +163,15 @@ int path_mode(int32_t blk_h, int32_t blk_w) {
   return kPathAuto;
 }
 
+// DTC_METCF_STREAMS=1 runs the whole build on the caller's stream (no side
+// stream for the global path): the same kernels in the same order, one after
+// another, as a caller needing single-stream work may want, and to time the
+// kernels without overlap.
+bool one_stream() {
+  const char *e = std::getenv("DTC_METCF_STREAMS");
+  return e && !std::strcmp(e, "1");
+}
+
 __global__ void k_header_init(MetcfHeader *h) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     h->flags = 0;
@@ -1536,7 +1545,7 @@ dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *no
   for (int c = 0; c < kClasses; ++c) n_lds += host.cls_count[c];
   SideStream *side = nullptr;
   hipStream_t gs = stream;
-  if (nnz_big > 0 && n_lds > 0) {
+  if (nnz_big > 0 && n_lds > 0 && !one_stream()) {
     side = dev_state;
     gs = side->s;
   } else {

@@ -145,6 +145,24 @@ def test_dense_large_windows(ext, path, kind):
     check(ext, rowptr, col, M, ctx=f"dense large windows {kind}")
 
 
+def test_one_stream(ext, monkeypatch):
+    """DTC_METCF_STREAMS=1: the whole build on the caller's stream, bit-exact
+    on matrices with windows for every LDS class and for the device-wide
+    sort, unsorted rows and duplicates included."""
+    from dtc.graphs import rmat, sbm_hubs
+    monkeypatch.setenv("DTC_METCF_STREAMS", "1")
+    r = rmat(14, 16, seed=3)
+    check(ext, r.rowptr, r.col, r.num_rows, ctx="one stream rmat14")
+    b = sbm_hubs(20000, communities=20, avg_per_row=40, hubs=6, hub_min=3000, hub_max=20000, seed=4)
+    check(ext, b.rowptr, b.col, b.num_rows, ctx="one stream sbm")
+    rng = np.random.default_rng(13)
+    rows = [rng.permutation(rng.choice(1 << 16, 1500, replace=False)) for _ in range(16)]
+    rows += [rng.choice(1 << 16, int(rng.integers(1, 40)), replace=True) for _ in range(300)]
+    rowptr = np.concatenate([[0], np.cumsum([x.shape[0] for x in rows])]).astype(np.int32)
+    col = np.concatenate(rows).astype(np.int32)
+    check(ext, rowptr, col, len(rows), ctx="one stream unsorted+dups")
+
+
 def test_empty_and_degenerate(ext, path):
     for M in [0, 1, 15, 16, 17, 100]:
         rowptr = np.zeros(M + 1, np.int32)
