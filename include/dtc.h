@@ -117,7 +117,8 @@ dtc_status_t dtc_metcf_workspace_size(int32_t num_nodes, int32_t nnz, int32_t bl
  * block count), as the reference does (DTCSpMM_kernel.cu:378-380, :384-387).
  * Windows of more than 16,384 entries are sorted on a stream the library
  * creates once per device (default priority), forked from and joined back to
- * `stream` by events; builds on one device take turns at it under a lock.
+ * `stream` by events; builds on one device take turns at it under a lock
+ * (environment DTC_METCF_STREAMS=1: every kernel on `stream`).
  */
 dtc_status_t dtc_metcf_build_windows(const int32_t *edge_list, const int32_t *node_pointer,
                                      int32_t num_nodes, int32_t nnz, int32_t blk_h,
